@@ -1,0 +1,181 @@
+"""Benchmark: parallel-beam projector fwd+back throughput on one MI355X.
+
+Workload (the GPU configuration of the headline metric): 720 views over
+[0, pi), detector 64 rows x 512 channels, volume 512 x 512 x 64 FP32, seeded
+40-ellipsoid phantom, sinogram = its forward projection + 1% Gaussian noise.
+A step is one full forward_project plus one full back_project through the
+public model API with device-resident inputs (output_sharded=True).  The
+value is the whole-grid work, 2 * rows*cols*slices*views voxel-views per
+step, divided by the step time: Gvox.views/s.  Ten seeded VCD iterations on
+the same data are timed once after the projector steps (vcd_10it_ms).
+
+The roofline entry is for the dominant kernel of the step, timed live with
+HIP events on the stream it runs on.  The CPU baseline times the oracle
+(test infrastructure, FP64 sparse-matrix restatement of the reference) on a
+bounded sample of the same workload on this host.
+
+Prints ONE JSON line.  Only --gpus 1 is supported (one GPU per box).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+V, R, C = 720, 64, 512
+NX = 512
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--vcd-iters', type=int, default=10)
+    ap.add_argument('--no-vcd', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--cpu-views', type=int, default=24,
+                    help='views of the workload the CPU baseline projects')
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
+def cpu_baseline(angles, vol, sino, views):
+    """Oracle fwd+back on `views` views of the workload (FP64, 1 core)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import parallel as op
+    rs = vol.shape
+    idx = op.full_indices(rs)
+    geom = dict(num_rows=rs[0], num_cols=rs[1], num_channels=C, delta_det_channel=1.0,
+                det_channel_offset=0.0, delta_voxel=1.0, delta_voxel_row=1.0)
+    sel = np.linspace(0, V - 1, views).astype(int)
+    vals = vol.reshape(-1, rs[2])[idx]
+    t0 = time.perf_counter()
+    op.forward(vals, idx, angles[sel], geom)
+    op.back(sino[sel], idx, angles[sel], geom)
+    dt = time.perf_counter() - t0
+    work = 2.0 * rs[0] * rs[1] * rs[2] * views
+    return dict(value=work / dt / 1e9, unit='Gvox*views/s', cores=1, kind='port',
+                sample=f'oracle forward+back (FP64 sparse matrix, scipy, 1 thread) of {views} '
+                       f'of the {V} views, full 512x512x64 grid: {dt:.1f} s')
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1 '
+                                   '(multi-GPU sharding is designed, not built)'}))
+        sys.exit(2)
+    import torch
+    import mbirtorch_amd as mb
+    from mbirtorch_amd.phantom import random_ellipsoids, add_noise
+    from mbirtorch_amd import _lib
+
+    torch.cuda.set_device(0)
+    angles = np.linspace(0, np.pi, V, endpoint=False)
+    model = mb.ParallelBeamModel((V, R, C), angles)
+    rs = tuple(model.get_params('recon_shape'))
+    log(f'phantom {rs}')
+    vol_np = random_ellipsoids(rs, 40, seed=0)
+    vol = torch.as_tensor(vol_np, device='cuda')
+    clean = model.forward_project(vol, output_sharded=True)
+    sino_np = add_noise(clean.cpu().numpy(), 0.01, seed=1)
+    sino = torch.as_tensor(sino_np, device='cuda')
+    torch.cuda.synchronize()
+
+    def step():
+        model.forward_project(vol, output_sharded=True)
+        model.back_project(sino, output_sharded=True)
+
+    log(f'warmup {args.warmup}')
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f'timing {args.steps} steps')
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        step()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    work = 2.0 * rs[0] * rs[1] * rs[2] * V
+    value = work / (ms_step * 1e-3) / 1e9
+
+    # Dominant kernels, timed alone on the launch stream.
+    ops = model.projector()
+    idx = model.full_indices_device()
+    vals = vol.reshape(-1, rs[2])[idx].contiguous()
+    sino_vcr = model._vrc_to_vcr(sino)
+
+    def time_kernel(fn, n=5):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(n):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / n
+
+    fwd_ms = time_kernel(lambda: ops.forward(vals, idx, layout=_lib.MBIR_LAYOUT_VRC))
+    back_ms = time_kernel(lambda: ops.back(sino_vcr, idx))
+    P = int(idx.shape[0])
+    ror_vv = P * rs[2] * V
+    # Algorithmic FLOPs per ROR voxel-view: two taps x (mul + add).
+    flops = 4.0 * ror_vv
+    dom_name, dom_ms = ('back', back_ms) if back_ms >= fwd_ms else ('forward', fwd_ms)
+    achieved = flops / (dom_ms * 1e-3) / 1e12
+    bytes_alg = 4.0 * (P * rs[2] + V * R * C)
+    roofline = dict(bound='mfma', achieved=round(achieved, 3), peak=FP32_PEAK_TFLOPS,
+                    unit='TFLOP/s', frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=None,
+                    kernel=f'par_{dom_name}', hbm_gbs=round(bytes_alg / (dom_ms * 1e-3) / 1e9, 1))
+
+    vcd_ms = None
+    if not args.no_vcd:
+        log('vcd')
+        np.random.seed(0)
+        model.set_params(p=1.2, q=2.0, T=1.0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        model.recon(sino, max_iterations=args.vcd_iters, stop_threshold_change_pct=0.0,
+                    print_logs=False, logfile_path=None, output_sharded=True)
+        torch.cuda.synchronize()
+        vcd_ms = (time.perf_counter() - t1) * 1e3
+
+    cpu = None
+    if not args.no_cpu:
+        log('cpu baseline')
+        cpu = cpu_baseline(angles, vol_np, sino_np, args.cpu_views)
+
+    out = {
+        'metric': 'Gvox*views/s fwd+back (parallel 720v 64x512 det, 512x512x64 vol)',
+        'value': round(value, 2), 'unit': 'Gvox*views/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(ms_step, 4),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+        'data': 'synthetic (seeded 40-ellipsoid phantom, forward projection + 1% noise)',
+        'config': {'workload': 'parallel-beam fwd+back: 720 views [0,pi), 64x512 detector, '
+                               '512x512x64 volume', 'views': V, 'det_rows': R,
+                   'det_channels': C, 'volume': list(rs), 'parallelism': 'single GPU'},
+        'roofline': roofline, 'cpu_baseline': cpu,
+        'fwd_ms': round(fwd_ms, 4), 'back_ms': round(back_ms, 4),
+        'vcd_iters': None if vcd_ms is None else args.vcd_iters,
+        'vcd_ms': None if vcd_ms is None else round(vcd_ms, 1),
+        'wall_s': round(wall, 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

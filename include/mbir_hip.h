@@ -1,0 +1,228 @@
+/*
+ * mbir_hip.h -- C-ABI of the MI355X (gfx950) projector / VCD library
+ * (libmbir_hip.so).
+ *
+ * Conventions
+ *  - Every pointer is a device pointer owned by the caller; the library never
+ *    allocates.  Scratch arrays (partials) are passed in, sized by the
+ *    *_count / *_bytes queries below.
+ *  - Every call is asynchronous on `stream` (a hipStream_t) for the current
+ *    device; the caller sets the device.
+ *  - Sinogram-like arrays are float32.  `layout` MBIR_LAYOUT_VRC is the
+ *    reference's (views, rows, channels); MBIR_LAYOUT_VCR is channel-major
+ *    (views, channels, rows), the library's internal layout (the reference
+ *    bodies transpose to it on every call: horizontal_fan.py:38-43,
+ *    triton_parallel.py:314-330).
+ *  - Volumes are flat (rows*cols, slices), the reference's flat_recon
+ *    (tomography_model.py:1714-1760); voxel cylinders at a pixel list are
+ *    (num_pixels, slices).
+ *  - Return 0 on success, a negative MBIR_ERR_* code otherwise; no call
+ *    touches memory after a non-zero return.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference checkout).
+ */
+#ifndef MBIR_HIP_H
+#define MBIR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBIR_OK 0
+#define MBIR_ERR_SHAPE (-1)
+#define MBIR_ERR_GEOMETRY (-2)
+#define MBIR_ERR_LAUNCH (-3)
+#define MBIR_ERR_WORKSPACE (-4)
+#define MBIR_ERR_ARG (-5)
+
+#define MBIR_LAYOUT_VRC 0
+#define MBIR_LAYOUT_VCR 1
+
+typedef void* mbir_stream_t; /* hipStream_t */
+
+/* Parallel-beam geometry: the scalars of _view_batch_args
+ * (parallel_beam.py:278-291) plus derived constants.  Per-view constants
+ * (cos, sin, W, ws, (W+1)/2, min(W,1), 0, 0) come in a separate float32
+ * table of 8 floats per view, computed on the host with the reference's
+ * float32 operation order (parallel_beam.py:38-51). */
+typedef struct {
+  int32_t num_rows;      /* recon rows */
+  int32_t num_cols;      /* recon columns */
+  int32_t num_slices;    /* recon slices == detector rows */
+  int32_t num_channels;  /* detector channels */
+  int32_t num_views;
+  int32_t psf_radius;    /* reference tap radius (parallel_beam.py:172-180) */
+  int32_t two_tap;       /* 1 when every view has W <= 1: the two taps
+                            floor(n_p), floor(n_p)+1 hold every nonzero weight */
+  int32_t unit_channel;  /* 1 when delta_det_channel == 1 (division is exact) */
+  float delta_det_channel;
+  float det_channel_offset;
+  float delta_voxel;
+  float delta_voxel_row;
+  float det_center;      /* (C - 1) / 2 */
+  float row_center;      /* (num_rows - 1) / 2 */
+  float col_center;      /* (num_cols - 1) / 2 */
+  float pad;
+} mbir_par_geom;
+
+/* qGGMRF / VCD scalars (qggmrf.py:22-74, tomography_model.py:2463-2472). */
+typedef struct {
+  float b[6];            /* row+, row-, col+, col-, slice+, slice- (normalised) */
+  float two_b[6];        /* float32(b[i] * 2.0) as the reference forms it */
+  float sigma_x, p, q, T;
+  float a_min;           /* float32(T * sigma_x * eps32) */
+  float t_sigma;         /* float32(T * sigma_x) */
+  float q_minus_p, p_minus_2, q_over_p;
+  float two_sigma_p;     /* float32(2 * sigma_x ** p) */
+  float fm_constant;     /* float32(1 / sigma_y**2) */
+  float neg_fm_constant; /* float32(-(1 / sigma_y**2)) */
+  float inv_sigma_prox2; /* float32(1 / sigma_prox**2) (proximal-map prior) */
+  float max_alpha;
+  int32_t num_rows, num_cols, num_slices;
+  int32_t use_prox;      /* 1: proximal-map prior instead of qGGMRF */
+} mbir_vcd_params;
+
+/* ---- library info ------------------------------------------------------ */
+int mbir_version(void);
+
+/* ---- parallel-beam projectors ----------------------------------------- */
+
+/* Sparse forward projection of voxel cylinders values (num_pixels, slices)
+ * at pixel_indices into a full sinogram, written (not accumulated) in
+ * `layout`.  Replaces the forward body _parallel_forward_view_batch
+ * (parallel_beam.py:56-73; Triton body triton_parallel.py:598-745) driven
+ * over the whole view range by Projectors.sparse_forward_project_view_range
+ * (projectors.py:518-588). */
+int mbir_par_forward(const float* values, const int64_t* pixel_indices,
+                     int32_t num_pixels, const float* view_table,
+                     const mbir_par_geom* geom, int32_t layout,
+                     float* sino_out, mbir_stream_t stream);
+
+/* Number of view splits mbir_par_back uses for num_pixels (>= 1). */
+int32_t mbir_par_back_splits(const mbir_par_geom* geom, int32_t num_pixels);
+
+/* Sparse back projection of a channel-major sinogram (optionally multiplied
+ * by weights of the same layout) onto voxel cylinders, with the weights
+ * raised to coeff_power (1, or 2 for the Hessian diagonal).  Writes
+ * num_splits partial sums (num_splits, num_pixels, slices) over disjoint view
+ * ranges into `partials`; num_splits == 1 writes the result itself.
+ * Replaces _parallel_back_view_batch (parallel_beam.py:76-94; Triton body
+ * triton_parallel.py:250-369) driven by
+ * Projectors.sparse_back_project_view_range (projectors.py:590-646). */
+int mbir_par_back(const float* sino_vcr, const float* weights_vcr,
+                  const int64_t* pixel_indices, int32_t num_pixels,
+                  const float* view_table, const mbir_par_geom* geom,
+                  int32_t coeff_power, int32_t num_splits, float* partials,
+                  mbir_stream_t stream);
+
+/* out[i] = sum_k parts[k * n + i], k ascending (deterministic). */
+int mbir_reduce_splits(const float* parts, int32_t num_splits, int64_t n,
+                       float* out, mbir_stream_t stream);
+
+/* Number of FP64 partial pairs the fused forward+line-search writes. */
+int32_t mbir_par_fwd_partials_count(const mbir_par_geom* geom);
+
+/* Forward projection of the subset direction delta fused with the line
+ * search reductions lin = sum(w * e * A delta), quad = sum(w * (A delta)^2)
+ * (w == NULL means unit weights), FP64, one (lin, quad) pair per workgroup
+ * into partials[2 * mbir_par_fwd_partials_count].  Replaces
+ * sparse_forward_project(delta) + _forward_lin_quad_const/_weighted
+ * (tomography_model.py:2592-2610, 65-74). */
+int mbir_par_fwd_linesearch(const float* delta, const int64_t* pixel_indices,
+                            int32_t num_pixels, const float* view_table,
+                            const mbir_par_geom* geom, const float* error_vcr,
+                            const float* weights_vcr, double* partials,
+                            mbir_stream_t stream);
+
+/* error -= alpha[0] * (A delta), recomputing A delta tile by tile (no
+ * sinogram-sized transient).  Replaces error_sinogram.sub_(alpha *
+ * delta_sinogram) in _apply_update (tomography_model.py:77-87). */
+int mbir_par_fwd_update(const float* delta, const int64_t* pixel_indices,
+                        int32_t num_pixels, const float* view_table,
+                        const mbir_par_geom* geom, const float* alpha,
+                        float* error_vcr, mbir_stream_t stream);
+
+/* ---- VCD subset step --------------------------------------------------- */
+
+/* Number of FP64 partial pairs mbir_vcd_direction writes. */
+int32_t mbir_vcd_direction_partials_count(int32_t num_pixels,
+                                          int32_t num_slices);
+
+/* Prior gradient/Hessian at the subset (qGGMRF surrogate, or the proximal
+ * prior when prm->use_prox), forward gradient -fm*sum(back partials) and
+ * Hessian fm*fm_hessian[pix], the direction delta = -(g_f+g_p)/(H_f+H_p),
+ * and the prior line terms (sum g_p*delta, sum H_p*delta^2) as FP64 partial
+ * pairs.  Replaces qggmrf_gradient_and_hessian_at_indices
+ * (qggmrf.py:77-183), prox_gradient_at_indices (qggmrf.py:186-206),
+ * _diagonal_update_direction and _prior_line_terms
+ * (tomography_model.py:54-62, 2520-2576). */
+int mbir_vcd_direction(const float* flat_recon, const int64_t* pixel_indices,
+                       int32_t num_pixels, const float* back_parts,
+                       int32_t num_splits, const float* fm_hessian,
+                       const float* prox_input, const mbir_vcd_params* prm,
+                       float* delta, double* partials, mbir_stream_t stream);
+
+/* alpha = clamp((fm*lin - prior_lin) / (fm*quad + prior_quad + eps32),
+ * eps32, max_alpha), reduced in a fixed order; writes alpha[0] (float32)
+ * and adds it to alpha_sum[0] (FP64).  Replaces tomography_model.py:2612-2617. */
+int mbir_vcd_alpha(const double* prior_parts, int32_t num_prior,
+                   const double* fwd_parts, int32_t num_fwd,
+                   const mbir_vcd_params* prm, float* alpha,
+                   double* alpha_sum, mbir_stream_t stream);
+
+/* delta = max(-x[pix] / (alpha + eps32), delta).  tomography_model.py:2624-2633. */
+int mbir_vcd_positivity(const float* flat_recon, const int64_t* pixel_indices,
+                        int32_t num_pixels, int32_t num_slices,
+                        const float* alpha, float* delta, mbir_stream_t stream);
+
+/* x[pix] += alpha*delta, then acc[s] += sum_p (alpha*delta[p, s])^2 for
+ * each slice s and acc[num_slices] += sum |alpha*delta|, FP64, each sum owned
+ * by one workgroup (deterministic).  Replaces _apply_update's recon half and
+ * its per-slice / ell1 reductions (tomography_model.py:77-87). */
+int mbir_vcd_apply(float* flat_recon, const int64_t* pixel_indices,
+                   int32_t num_pixels, int32_t num_slices, const float* delta,
+                   const float* alpha, double* acc, mbir_stream_t stream);
+
+/* ---- reductions -------------------------------------------------------- */
+
+/* Scratch doubles the reductions need: 3 * mbir_reduce_blocks(n). */
+int32_t mbir_reduce_blocks(int64_t n);
+
+/* out[0] = sum w*a*a, out[1] = sum a*a, out[2] = sum w (w NULL: 1), FP64.
+ * Backs get_forward_model_loss / _vcd_iteration_stats
+ * (tomography_model.py:2323-2357). */
+int mbir_sumsq_f64(const float* a, const float* w, int64_t n, double* scratch,
+                   double* out, mbir_stream_t stream);
+
+/* out[0] = sum |a|, FP64 (image_ell1, _memory_ledger.py:134-163). */
+int mbir_sumabs_f64(const float* a, int64_t n, double* scratch, double* out,
+                    mbir_stream_t stream);
+
+/* out[0] = sum a*b, FP64 (adjoint inner products). */
+int mbir_dot_f64(const float* a, const float* b, int64_t n, double* scratch,
+                 double* out, mbir_stream_t stream);
+
+/* ---- direct-reconstruction filter and layout -------------------------- */
+
+/* Per-row 'valid' convolution with the 2C-1 taps (already scaled), with an
+ * optional (rows, channels) pre-weight; input in VRC layout, output in
+ * `out_layout`.  Replaces apply_row_filter (tomography_utils.py:29-78). */
+int mbir_row_filter(const float* sino_vrc, const float* taps,
+                    const float* row_weight, int32_t num_views,
+                    int32_t num_rows, int32_t num_channels, int32_t out_layout,
+                    float* out, mbir_stream_t stream);
+
+/* VRC <-> VCR transpose; to_vcr != 0 converts VRC -> VCR. */
+int mbir_transpose_sino(const float* in, int32_t num_views, int32_t num_rows,
+                        int32_t num_channels, int32_t to_vcr, float* out,
+                        mbir_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MBIR_HIP_H */

@@ -1,0 +1,181 @@
+"""Model parameters: the reference's names, defaults and set/get semantics.
+
+Names, default values and recompile flags follow _utils.py:56-111; the
+set_params special cases follow parameter_handler.py:260-344:
+* directly setting sigma_y / sigma_x / sigma_prox disables
+  auto-regularization (with a warning) unless no_warning;
+* setting sharpness or snr_db re-enables a disabled auto-regularization;
+* an unknown name raises ValueError, except under no_warning, where it is
+  added (how a geometry's own parameters enter);
+* get_params on an unknown name raises NameError.
+"""
+
+import copy
+import io
+import logging
+import os
+import warnings
+from dataclasses import dataclass
+from typing import Any
+
+import numpy as np
+
+
+@dataclass
+class Param:
+    val: Any
+    recompile_flag: bool = True
+
+
+def _defaults():
+    d = {
+        # forward model
+        'geometry_type': Param(None, False),
+        'file_format': Param(1.0, False),
+        'sinogram_shape': Param(None, True),
+        'delta_det_channel': Param(1.0, True),
+        'delta_det_row': Param(1.0, True),
+        'det_row_offset': Param(0.0, True),
+        'det_channel_offset': Param(0.0, True),
+        'sigma_y': Param(1.0, False),
+        'alu_unit': Param(None, False),
+        'alu_value': Param(1.0, False),
+        # recon model
+        'recon_shape': Param(None, True),
+        'delta_voxel': Param(None, True),
+        'voxel_row_aspect': Param(1.0, True),
+        'voxel_slice_aspect': Param(1.0, True),
+        'sigma_x': Param(1.0, False),
+        'sigma_prox': Param(1.0, False),
+        'p': Param(2.0, False),
+        'q': Param(1.2, False),
+        'T': Param(1.0, False),
+        'qggmrf_nbr_wts': Param([1.0, 1.0, 1.0], False),
+        # reconstruction
+        'auto_regularize_flag': Param(True, False),
+        'positivity_flag': Param(False, False),
+        'snr_db': Param(30.0, False),
+        'sharpness': Param(1.0, False),
+        'granularity': Param([1, 2, 4, 8, 16, 32, 64, 128, 128, 128, 128], False),
+        'partition_sequence': Param([2, 4, 6] + [7, 8, 9, 10] * 25, False),
+        'verbose': Param(1, False),
+        'max_alpha': Param(1.5, False),
+        'use_ror_mask': Param(True, False),
+    }
+    return d
+
+
+RECON_PARAM_NAMES = ['num_iterations', 'granularity', 'partition_sequence', 'fm_rmse',
+                     'prior_loss', 'regularization_params', 'stop_threshold_change_pct',
+                     'alpha_values', 'delta_norm_per_slice']
+
+AUTO_REGULARIZATION_PARAM_NAMES = ('sigma_y', 'sigma_x', 'sigma_prox')
+
+
+class ParameterHandler:
+    """Parameter store + per-model logger (parameter_handler.py:27-344)."""
+
+    def __init__(self):
+        self.params = _defaults()
+        self.log_buffer = None
+        self._log_file_handler = None
+        self.logger = logging.getLogger(f'mbirtorch_amd.model.{id(self)}')
+        self.logger.propagate = False
+        self.logger.setLevel(logging.INFO)
+
+    # -- logging ----------------------------------------------------------------
+    def setup_logger(self, logfile_path=None, print_logs=True):
+        for h in list(self.logger.handlers):
+            self.logger.removeHandler(h)
+            h.close()
+        self.log_buffer = io.StringIO()
+        fmt = logging.Formatter('%(message)s')
+        bh = logging.StreamHandler(self.log_buffer)
+        bh.setFormatter(fmt)
+        self.logger.addHandler(bh)
+        if print_logs:
+            ch = logging.StreamHandler()
+            ch.setFormatter(fmt)
+            self.logger.addHandler(ch)
+        self._log_file_handler = None
+        if logfile_path:
+            path = os.path.abspath(os.path.expanduser(logfile_path))
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            fh = logging.FileHandler(path, mode='w')
+            fh.setFormatter(fmt)
+            self.logger.addHandler(fh)
+            self._log_file_handler = fh
+
+    def close_log_file(self):
+        fh = self._log_file_handler
+        if fh is not None:
+            self.logger.removeHandler(fh)
+            fh.close()
+            self._log_file_handler = None
+
+    # -- access -----------------------------------------------------------------
+    def get_params(self, parameter_names):
+        if isinstance(parameter_names, str):
+            if parameter_names not in self.params:
+                raise NameError(f"'{parameter_names}' not a recognized parameter")
+            return self.params[parameter_names].val
+        out = []
+        for name in parameter_names:
+            if name not in self.params:
+                raise NameError(f"'{name}' not a recognized parameter")
+            out.append(self.params[name].val)
+        return out
+
+    def set_params(self, no_warning=False, no_compile=False, **kwargs):
+        recompile = False
+        reg_change = False
+        meta_change = False
+        for key, val in kwargs.items():
+            flag = True
+            if key in self.params:
+                flag = self.params[key].recompile_flag
+            elif not no_warning:
+                raise ValueError(f'{key} is not a recognized parameter\nValid parameters are: \n'
+                                 + ''.join(f'   {k}\n' for k in self.params))
+            if isinstance(val, np.generic):
+                val = val.item()
+            self.params[key] = Param(val, flag)
+            if flag:
+                recompile = True
+            elif key in AUTO_REGULARIZATION_PARAM_NAMES:
+                reg_change = True
+            elif key in ('sharpness', 'snr_db'):
+                meta_change = True
+        if reg_change and not no_warning:
+            self.set_params(auto_regularize_flag=False)
+            warnings.warn('You are directly setting regularization parameters, sigma_x, '
+                          'sigma_y or sigma_prox. This is an advanced feature that will '
+                          'disable auto-regularization.')
+        if meta_change and self.get_params('auto_regularize_flag') is False:
+            self.set_params(auto_regularize_flag=True)
+            if not no_warning:
+                warnings.warn('You have re-enabled auto-regularization by setting sharpness '
+                              'or snr_db. It was previously disabled')
+        if recompile and not no_compile:
+            self.refresh_device_bindings()
+
+    def refresh_device_bindings(self):
+        pass
+
+    def verify_valid_params(self):
+        shape = self.get_params('sinogram_shape')
+        if shape is None or len(shape) != 3:
+            raise ValueError(f'sinogram_shape must be (views, rows, channels); got {shape}')
+
+    def get_params_dict(self):
+        return {k: copy.deepcopy(v.val) for k, v in self.params.items()}
+
+    def print_params(self):
+        print('----')
+        for k, v in self.params.items():
+            val = v.val
+            if isinstance(val, np.ndarray) and val.size > 16:
+                print(f'{k} = array(shape={val.shape}, dtype={val.dtype})')
+            else:
+                print(f'{k} = {val}')
+        print('----')

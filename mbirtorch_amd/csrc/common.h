@@ -1,0 +1,66 @@
+// Shared device helpers for the MI355X (gfx950) projector / VCD kernels.
+//
+// All kernels are written for 64-wide wavefronts.  Geometry arithmetic is
+// compiled with -ffp-contract=off so every add/mul/div rounds on its own, in
+// the order the reference's eager float32 chain uses (parallel_beam.py:22-53,
+// cone_beam.py:59-158); accumulations call fmaf() explicitly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mbir_hip.h"
+
+#define MBIR_WAVE 64
+
+#define MBIR_CHECK_LAUNCH()                                   \
+  do {                                                        \
+    hipError_t e_ = hipGetLastError();                        \
+    if (e_ != hipSuccess) return MBIR_ERR_LAUNCH;             \
+  } while (0)
+
+namespace mbir {
+
+// Per-view constants of the parallel chain, computed on the host in float32
+// exactly as the reference's eager torch ops do (cos/sin of the float32
+// angle, footprint, W, weight scale), then uploaded once per model.
+struct ParView {
+  float cosv, sinv;   // cos/sin of the view angle
+  float W;            // projected voxel width in channel units
+  float ws;           // weight scale = dv*dvr / footprint
+  float halfw;        // (W + 1) / 2
+  float lmax;         // min(W, 1)
+  float pad0, pad1;
+};
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, MBIR_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, MBIR_WAVE);
+  return v;
+}
+
+// Block-wide FP64 sum of two values; result valid in thread 0.  Requires
+// blockDim.x a multiple of 64 and <= 1024.  Deterministic: fixed tree.
+__device__ __forceinline__ void block_sum2_f64(double& a, double& b) {
+  __shared__ double red_a[16], red_b[16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  a = wave_sum_f64(a);
+  b = wave_sum_f64(b);
+  __syncthreads();
+  if (lane == 0) { red_a[wid] = a; red_b[wid] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+    const int nw = blockDim.x >> 6;
+    for (int i = 0; i < nw; ++i) { sa += red_a[i]; sb += red_b[i]; }
+    a = sa; b = sb;
+  }
+}
+
+}  // namespace mbir

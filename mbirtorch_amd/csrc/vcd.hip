@@ -1,0 +1,323 @@
+// VCD subset-step kernels for MI355X (gfx950): prior + direction, the
+// on-device step size, positivity, the recon update, and FP64 reductions.
+//
+// Elementwise float32 arithmetic follows the reference's order
+// (qggmrf.py:35-183, tomography_model.py:54-87, 2508-2657); every sum that
+// decides the step size or a reported statistic is accumulated in FP64 with
+// a fixed reduction tree, so a subset step is deterministic and needs no
+// host synchronisation (alpha stays on the device).
+#include "common.h"
+
+namespace mbir {
+
+constexpr int kThreads = 256;
+constexpr float kEps32 = 1.1920928955078125e-07f;
+
+// 2*b_tilde(delta) without the direction weight: rho'(delta)/delta of the
+// qGGMRF potential with |delta| floored at a_min (qggmrf.py:35-57).
+__device__ __forceinline__ float qg_b_tilde(float delta, const mbir_vcd_params& p) {
+  const float ad = fmaxf(fabsf(delta), p.a_min);
+  const float ds = ad / p.t_sigma;
+  const float u = powf(ds, p.q_minus_p);
+  const float pm2 = powf(ad, p.p_minus_2);
+  float num = pm2 / p.two_sigma_p;
+  num = (num * u) * (p.q_over_p + u);
+  const float den = 1.0f + u;
+  return num / (den * den);
+}
+
+__global__ __launch_bounds__(kThreads) void vcd_direction_kernel(
+    const float* __restrict__ x, const int64_t* __restrict__ pix, int P,
+    const float* __restrict__ back_parts, int nsplit, const float* __restrict__ hess,
+    const float* __restrict__ prox, mbir_vcd_params prm, float* __restrict__ delta,
+    double* __restrict__ partials) {
+  const int S = prm.num_slices, Nc = prm.num_cols, Nr = prm.num_rows;
+  const int64_t total = (int64_t)P * S;
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  double pl = 0.0, pq = 0.0;
+  if (i < total) {
+    const int p = (int)(i / S), s = (int)(i - (int64_t)p * S);
+    const int64_t pi = pix[p];
+    const float xs = x[pi * S + s];
+    float gp, hp;
+    if (prm.use_prox) {
+      gp = prm.inv_sigma_prox2 * (xs - prox[pi * S + s]);
+      hp = prm.inv_sigma_prox2;
+    } else {
+      // Slice neighbours: deltas v[s]-v[s-1] and v[s+1]-v[s], zero at the
+      // reflected ends (qggmrf.py:135-156).
+      const float dm = s > 0 ? xs - x[pi * S + s - 1] : 0.0f;
+      const float dp = s < S - 1 ? x[pi * S + s + 1] - xs : 0.0f;
+      const float cm = 2.0f * qg_b_tilde(dm, prm);
+      const float cp = 2.0f * qg_b_tilde(dp, prm);
+      gp = (-prm.b[4]) * (cp * dp) + prm.b[5] * (cm * dm);
+      hp = prm.b[4] * cp + prm.b[5] * cm;
+      // In-plane neighbours, indices clamped to the grid (qggmrf.py:158-181).
+      const int row = (int)(pi / Nc), col = (int)(pi - (int64_t)row * Nc);
+      const int nbr[4] = {min(row + 1, Nr - 1) * Nc + col, max(row - 1, 0) * Nc + col,
+                          row * Nc + min(col + 1, Nc - 1), row * Nc + max(col - 1, 0)};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float d = xs - x[(int64_t)nbr[k] * S + s];
+        const float c2 = prm.two_b[k] * qg_b_tilde(d, prm);
+        gp = gp + c2 * d;
+        hp = hp + c2;
+      }
+    }
+    float bsum = back_parts[i];
+    for (int k = 1; k < nsplit; ++k) bsum += back_parts[(size_t)k * total + i];
+    const float fg = prm.neg_fm_constant * bsum;
+    const float fh = prm.fm_constant * hess[pi * S + s];
+    const float d = -((fg + gp) / (fh + hp));
+    delta[i] = d;
+    pl = (double)(gp * d);
+    pq = (double)((hp * d) * d);
+  }
+  block_sum2_f64(pl, pq);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = pl;
+    partials[2 * blockIdx.x + 1] = pq;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void vcd_alpha_kernel(
+    const double* __restrict__ pp, int npp, const double* __restrict__ fp, int nfp,
+    double fm, double max_alpha, float* __restrict__ alpha, double* __restrict__ alpha_sum) {
+  double pl = 0.0, pq = 0.0, lin = 0.0, quad = 0.0;
+  for (int i = threadIdx.x; i < npp; i += kThreads) { pl += pp[2 * i]; pq += pp[2 * i + 1]; }
+  for (int i = threadIdx.x; i < nfp; i += kThreads) { lin += fp[2 * i]; quad += fp[2 * i + 1]; }
+  block_sum2_f64(pl, pq);
+  block_sum2_f64(lin, quad);
+  if (threadIdx.x == 0) {
+    const double eps = (double)kEps32;
+    double a = (fm * lin - pl) / (fm * quad + pq + eps);
+    if (!(a >= eps)) a = eps;  // also maps NaN to eps, like a clamp of a NaN-free value
+    if (a > max_alpha) a = max_alpha;
+    const float af = (float)a;
+    alpha[0] = af;
+    alpha_sum[0] += (double)af;
+  }
+}
+
+__global__ void vcd_positivity_kernel(const float* __restrict__ x, const int64_t* __restrict__ pix,
+                                      int P, int S, const float* __restrict__ alpha,
+                                      float* __restrict__ delta) {
+  const float pc = 1.0f / (alpha[0] + kEps32);
+  const int64_t total = (int64_t)P * S;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int p = (int)(i / S), s = (int)(i - (int64_t)p * S);
+    delta[i] = fmaxf(-pc * x[pix[p] * S + s], delta[i]);
+  }
+}
+
+__global__ void vcd_apply_kernel(float* __restrict__ x, const int64_t* __restrict__ pix, int P,
+                                 int S, const float* __restrict__ delta,
+                                 const float* __restrict__ alpha) {
+  const float a = alpha[0];
+  const int64_t total = (int64_t)P * S;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int p = (int)(i / S), s = (int)(i - (int64_t)p * S);
+    // A subset may list a pixel twice (partition padding); both updates land,
+    // as with index_add_.  The duplicates carry identical deltas, so the two
+    // atomic adds commute exactly and the result stays deterministic.
+    atomicAdd(&x[pix[p] * S + s], a * delta[i]);
+  }
+}
+
+// Block s < S owns acc[s] = sum_p (alpha*delta[p,s])^2; block S owns the
+// ell1 sum acc[S] = sum |alpha*delta|.
+__global__ __launch_bounds__(kThreads) void vcd_stats_kernel(const float* __restrict__ delta,
+                                                             int P, int S,
+                                                             const float* __restrict__ alpha,
+                                                             double* __restrict__ acc) {
+  const float a = alpha[0];
+  const int s = blockIdx.x;
+  double v = 0.0, unused = 0.0;
+  if (s < S) {
+    for (int p = threadIdx.x; p < P; p += kThreads) {
+      const float ds = a * delta[(size_t)p * S + s];
+      v += (double)(ds * ds);
+    }
+  } else {
+    const int64_t total = (int64_t)P * S;
+    for (int64_t i = threadIdx.x; i < total; i += kThreads) v += (double)fabsf(a * delta[i]);
+  }
+  block_sum2_f64(v, unused);
+  if (threadIdx.x == 0) acc[s] += v;
+}
+
+// ---- reductions ------------------------------------------------------------
+constexpr int kRedItems = 8;  // elements per thread per block pass
+
+__global__ __launch_bounds__(kThreads) void sumsq_partial_kernel(const float* __restrict__ a,
+                                                                 const float* __restrict__ w,
+                                                                 int64_t n,
+                                                                 double* __restrict__ part) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    const float v = a[i];
+    const float vv = v * v;
+    if (w) {
+      const float wv = w[i];
+      s0 += (double)(vv * wv);
+      s2 += (double)wv;
+    } else {
+      s0 += (double)vv;
+      s2 += 1.0;
+    }
+    s1 += (double)vv;
+  }
+  double t = s2, unused = 0.0;
+  block_sum2_f64(s0, s1);
+  block_sum2_f64(t, unused);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = s0;
+    part[3 * blockIdx.x + 1] = s1;
+    part[3 * blockIdx.x + 2] = t;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void abs_dot_partial_kernel(const float* __restrict__ a,
+                                                                   const float* __restrict__ b,
+                                                                   int64_t n,
+                                                                   double* __restrict__ part) {
+  double s = 0.0, unused = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    s += b ? (double)a[i] * (double)b[i] : (double)fabsf(a[i]);
+  }
+  block_sum2_f64(s, unused);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = s;
+    part[3 * blockIdx.x + 1] = 0.0;
+    part[3 * blockIdx.x + 2] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void final3_kernel(const double* __restrict__ part, int nb,
+                                                          int nout, double* __restrict__ out) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, unused = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) {
+    s0 += part[3 * i];
+    s1 += part[3 * i + 1];
+    s2 += part[3 * i + 2];
+  }
+  block_sum2_f64(s0, s1);
+  block_sum2_f64(s2, unused);
+  if (threadIdx.x == 0) {
+    out[0] = s0;
+    if (nout > 1) out[1] = s1;
+    if (nout > 2) out[2] = s2;
+  }
+}
+
+static inline int red_blocks(int64_t n) {
+  int64_t b = (n + (int64_t)kThreads * kRedItems - 1) / ((int64_t)kThreads * kRedItems);
+  if (b < 1) b = 1;
+  if (b > 2048) b = 2048;
+  return (int)b;
+}
+
+}  // namespace mbir
+
+using namespace mbir;
+
+extern "C" {
+
+int32_t mbir_vcd_direction_partials_count(int32_t P, int32_t S) {
+  const int64_t total = (int64_t)P * S;
+  return (int32_t)((total + kThreads - 1) / kThreads);
+}
+
+int mbir_vcd_direction(const float* x, const int64_t* pix, int32_t P, const float* back_parts,
+                       int32_t nsplit, const float* hess, const float* prox,
+                       const mbir_vcd_params* prm, float* delta, double* partials,
+                       mbir_stream_t stream) {
+  if (!x || !pix || !back_parts || !hess || !prm || !delta || !partials) return MBIR_ERR_ARG;
+  if (prm->use_prox && !prox) return MBIR_ERR_ARG;
+  if (P <= 0 || nsplit < 1 || prm->num_slices <= 0) return MBIR_ERR_SHAPE;
+  const int nb = mbir_vcd_direction_partials_count(P, prm->num_slices);
+  vcd_direction_kernel<<<nb, kThreads, 0, (hipStream_t)stream>>>(x, pix, P, back_parts, nsplit,
+                                                                  hess, prox, *prm, delta,
+                                                                  partials);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int mbir_vcd_alpha(const double* pp, int32_t npp, const double* fp, int32_t nfp,
+                   const mbir_vcd_params* prm, float* alpha, double* alpha_sum,
+                   mbir_stream_t stream) {
+  if (!pp || !fp || !prm || !alpha || !alpha_sum || npp < 0 || nfp < 0) return MBIR_ERR_ARG;
+  vcd_alpha_kernel<<<1, kThreads, 0, (hipStream_t)stream>>>(
+      pp, npp, fp, nfp, (double)prm->fm_constant, (double)prm->max_alpha, alpha, alpha_sum);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int mbir_vcd_positivity(const float* x, const int64_t* pix, int32_t P, int32_t S,
+                        const float* alpha, float* delta, mbir_stream_t stream) {
+  if (!x || !pix || !alpha || !delta) return MBIR_ERR_ARG;
+  if (P <= 0 || S <= 0) return MBIR_ERR_SHAPE;
+  int nb = (int)(((int64_t)P * S + kThreads - 1) / kThreads);
+  if (nb > 8192) nb = 8192;
+  vcd_positivity_kernel<<<nb, kThreads, 0, (hipStream_t)stream>>>(x, pix, P, S, alpha, delta);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int mbir_vcd_apply(float* x, const int64_t* pix, int32_t P, int32_t S, const float* delta,
+                   const float* alpha, double* acc, mbir_stream_t stream) {
+  if (!x || !pix || !delta || !alpha || !acc) return MBIR_ERR_ARG;
+  if (P <= 0 || S <= 0) return MBIR_ERR_SHAPE;
+  hipStream_t s = (hipStream_t)stream;
+  int nb = (int)(((int64_t)P * S + kThreads - 1) / kThreads);
+  if (nb > 8192) nb = 8192;
+  vcd_apply_kernel<<<nb, kThreads, 0, s>>>(x, pix, P, S, delta, alpha);
+  MBIR_CHECK_LAUNCH();
+  vcd_stats_kernel<<<S + 1, kThreads, 0, s>>>(delta, P, S, alpha, acc);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int32_t mbir_reduce_blocks(int64_t n) { return red_blocks(n); }
+
+int mbir_sumsq_f64(const float* a, const float* w, int64_t n, double* scratch, double* out,
+                   mbir_stream_t stream) {
+  if (!a || !scratch || !out || n < 0) return MBIR_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = red_blocks(n);
+  sumsq_partial_kernel<<<nb, kThreads, 0, s>>>(a, w, n, scratch);
+  MBIR_CHECK_LAUNCH();
+  final3_kernel<<<1, kThreads, 0, s>>>(scratch, nb, 3, out);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int mbir_sumabs_f64(const float* a, int64_t n, double* scratch, double* out,
+                    mbir_stream_t stream) {
+  if (!a || !scratch || !out || n < 0) return MBIR_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = red_blocks(n);
+  abs_dot_partial_kernel<<<nb, kThreads, 0, s>>>(a, nullptr, n, scratch);
+  MBIR_CHECK_LAUNCH();
+  final3_kernel<<<1, kThreads, 0, s>>>(scratch, nb, 1, out);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int mbir_dot_f64(const float* a, const float* b, int64_t n, double* scratch, double* out,
+                 mbir_stream_t stream) {
+  if (!a || !b || !scratch || !out || n < 0) return MBIR_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = red_blocks(n);
+  abs_dot_partial_kernel<<<nb, kThreads, 0, s>>>(a, b, n, scratch);
+  MBIR_CHECK_LAUNCH();
+  final3_kernel<<<1, kThreads, 0, s>>>(scratch, nb, 1, out);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+"""CPU-only checks of the host side: the C-ABI library loads and exports every
+symbol include/mbir_hip.h declares; the model API's parameter semantics;
+the product's partition generator against the reference's golden partitions;
+the host view table; and loud failure without a GPU."""
+
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import mbirtorch_amd as mb
+from mbirtorch_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, 'include', 'mbir_hip.h')).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(mbir_[a-z0-9_]+)\s*\(', text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    syms = _declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.SIGNATURES), set(syms) ^ set(_lib.SIGNATURES)
+    assert lib.mbir_version() >= 1
+
+
+def test_geometry_struct_layout():
+    assert torch is not None
+    import ctypes
+    assert ctypes.sizeof(_lib.ParGeom) == 16 * 4
+    assert ctypes.sizeof(_lib.VcdParams) == (12 + 14) * 4 + 4 * 4
+
+
+def test_size_queries_need_no_gpu():
+    lib = _lib.load()
+    g = _lib.ParGeom(num_rows=512, num_cols=512, num_slices=64, num_channels=512,
+                     num_views=720, psf_radius=1, two_tap=1, unit_channel=1,
+                     delta_det_channel=1.0, delta_voxel=1.0, delta_voxel_row=1.0)
+    assert lib.mbir_par_back_splits(g, 205012) == 1
+    assert lib.mbir_par_back_splits(g, 1602) > 1
+    assert lib.mbir_par_fwd_partials_count(g) > 0
+    assert lib.mbir_reduce_blocks(10 ** 6) >= 1
+
+
+def test_model_geometry_matches_reference_rules():
+    angles = np.linspace(0, np.pi, 720, endpoint=False)
+    m = mb.ParallelBeamModel((720, 64, 512), angles)
+    assert tuple(m.get_params('recon_shape')) == (512, 512, 64)
+    assert m.get_psf_radius() == 1
+    assert m.get_params('delta_voxel') == 1.0
+    m = mb.ParallelBeamModel((180, 8, 128), np.linspace(0, np.pi, 180, endpoint=False))
+    assert tuple(m.get_params('recon_shape')) == (128, 128, 8)
+
+
+def test_param_semantics():
+    m = mb.ParallelBeamModel((10, 4, 16), np.linspace(0, np.pi, 10, endpoint=False))
+    assert m.get_params(['p', 'q', 'T']) == [2.0, 1.2, 1.0]
+    with pytest.raises(NameError):
+        m.get_params('nope')
+    with pytest.raises(ValueError):
+        m.set_params(nope=1)
+    with pytest.warns(UserWarning):
+        m.set_params(sigma_x=0.3)
+    assert m.get_params('auto_regularize_flag') is False
+    with pytest.warns(UserWarning):
+        m.set_params(sharpness=0.5)
+    assert m.get_params('auto_regularize_flag') is True
+    m.set_params(p=1.2, q=2.0)
+    assert m.get_params('p') == 1.2
+    with pytest.raises(ValueError):
+        mb.ParallelBeamModel((10, 4, 16), np.linspace(0, np.pi, 9))
+    m.set_params(voxel_slice_aspect=2.0)
+    with pytest.raises(ValueError):
+        m.verify_valid_params()
+
+
+def test_partitions_match_reference_golden():
+    g = np.load(os.path.join(ROOT, 'tests', 'golden', 'partitions.npz'), allow_pickle=False)
+    for seed, shape in ((0, (32, 32, 4)), (7, (40, 28, 2))):
+        np.random.seed(seed)
+        parts = mb.gen_set_of_pixel_partitions(shape, [1, 2, 4, 8, 16, 32, 64, 128, 128, 128,
+                                                       128])
+        for k, p in enumerate(parts):
+            assert np.array_equal(p.numpy(), g[f's{seed}_p{k}']), (seed, k)
+        assert np.array_equal(np.random.permutation(16), g[f's{seed}_next'])
+
+
+def test_partition_sequence():
+    seq = mb.gen_partition_sequence([2, 4, 6] + [7, 8, 9, 10] * 25, 10)
+    assert list(seq) == [2, 4, 6, 7, 8, 9, 10, 7, 8, 9]
+    assert list(mb.gen_partition_sequence([0, 1], 4)) == [0, 1, 1, 1]
+
+
+def test_view_table_uses_float32_chain():
+    from mbirtorch_amd.parallel_beam import parallel_view_table
+    angles = np.linspace(0, np.pi, 37, endpoint=False).astype(np.float32)
+    t = parallel_view_table(angles, 1.0, 0.8, 0.9).numpy()
+    c = np.cos(angles.astype(np.float64))
+    assert np.allclose(t[:, 0], c, atol=1e-6)
+    fp = np.maximum(np.abs(t[:, 0]) * np.float32(1.0), np.abs(t[:, 1]) * np.float32(0.8))
+    assert np.array_equal(t[:, 2], (fp / np.float32(0.9)).astype(np.float32))
+    assert np.array_equal(t[:, 4], ((t[:, 2] + np.float32(1)) / np.float32(2)))
+    assert np.array_equal(t[:, 5], np.minimum(t[:, 2], np.float32(1)))
+
+
+def test_autoregularization_matches_reference_golden():
+    g = np.load(os.path.join(ROOT, 'tests', 'golden', 'autoreg.npz'), allow_pickle=False)
+    for name in ('a', 'b'):
+        sino, angles = g[f'{name}_sino'], g[f'{name}_angles']
+        w = np.exp(-np.clip(sino, 0, None) * 10).astype(np.float32)
+        for tag, weights, sharp in (('u', None, 1.0), ('w', w, 0.7)):
+            m = mb.ParallelBeamModel(sino.shape, angles)
+            m.set_params(sharpness=sharp)
+            with np.errstate(all='ignore'):
+                import warnings
+                with warnings.catch_warnings():
+                    warnings.simplefilter('ignore')
+                    v = m.auto_set_regularization_params(sino, weights=weights)
+            got = [v['sigma_y'], v['sigma_x'], v['sigma_prox']]
+            np.testing.assert_allclose(got, g[f'{name}_{tag}'], rtol=1e-7)
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason='checks the no-GPU error path')
+def test_projection_without_gpu_fails_loudly():
+    m = mb.ParallelBeamModel((10, 4, 16), np.linspace(0, np.pi, 10, endpoint=False))
+    with pytest.raises(RuntimeError, match='HIP'):
+        m.forward_project(np.zeros(m.get_params('recon_shape'), np.float32))
+
+
+def test_phantom_is_seeded_and_leaves_global_rng_alone():
+    from mbirtorch_amd.phantom import random_ellipsoids
+    np.random.seed(5)
+    a = random_ellipsoids((32, 32, 8), 4, seed=1, axes_range=(2, 8))
+    after = np.random.random()
+    np.random.seed(5)
+    b = random_ellipsoids((32, 32, 8), 4, seed=1, axes_range=(2, 8))
+    assert np.array_equal(a, b) and np.random.random() == after
+    assert a.max() > 0 and a.min() >= 0
