@@ -129,7 +129,8 @@ def main():
         torch.cuda.synchronize()
         return a.elapsed_time(b) / n
 
-    fwd_ms = time_kernel(lambda: ops.forward(vals, idx, layout=_lib.MBIR_LAYOUT_VRC))
+    runs = model.full_index_runs_device()
+    fwd_ms = time_kernel(lambda: ops.forward(vals, idx, layout=_lib.MBIR_LAYOUT_VRC, runs=runs))
     back_ms = time_kernel(lambda: ops.back(sino_vcr, idx))
     P = int(idx.shape[0])
     ror_vv = P * rs[2] * V

@@ -69,6 +69,36 @@ typedef struct {
   float pad;
 } mbir_par_geom;
 
+/* Cone-beam geometry: the scalars of ConeBeamModel._view_batch_args
+ * (cone_beam.py:397-414) plus derived float32 constants.  Per-view constants
+ * (cos, sin, angle, z_offset = recon_slice_offset - z_shift) come in a float32
+ * table of 4 floats per view. */
+typedef struct {
+  int32_t num_rows;      /* recon rows */
+  int32_t num_cols;      /* recon columns */
+  int32_t num_slices;    /* recon slices */
+  int32_t num_det_rows;
+  int32_t num_channels;
+  int32_t num_views;
+  int32_t psf_radius;    /* channel taps and back vertical taps (cone_beam.py:519-553) */
+  int32_t bp_psf_radius; /* forward vertical taps */
+  int32_t curved;        /* 1: cylindrical detector of radius SDD */
+  int32_t unit_channel;  /* delta_det_channel == 1 */
+  int32_t pad0, pad1;
+  float delta_voxel, delta_voxel_row, delta_voxel_slice;
+  float delta_det_channel, delta_det_row;
+  float det_channel_offset, det_row_offset;
+  float source_detector_dist;  /* may be +inf */
+  float source_iso_dist;       /* float32(SDD / M) */
+  float inv_magnification;     /* float32(1 / M) */
+  float det_center_channel;    /* (C - 1) / 2 */
+  float det_center_row;        /* (R - 1) / 2 */
+  float row_center, col_center, slice_center; /* (N - 1) / 2 of the recon axes */
+  float half_slice_span;       /* float32(delta_voxel_slice * (S - 1) / 2) */
+  float area;                  /* float32(delta_voxel_row * delta_voxel) */
+  float pad2;
+} mbir_cone_geom;
+
 /* qGGMRF / VCD scalars (qggmrf.py:22-74, tomography_model.py:2463-2472). */
 typedef struct {
   float b[6];            /* row+, row-, col+, col-, slice+, slice- (normalised) */
@@ -93,14 +123,18 @@ int mbir_version(void);
 
 /* Sparse forward projection of voxel cylinders values (num_pixels, slices)
  * at pixel_indices into a full sinogram, written (not accumulated) in
- * `layout`.  Replaces the forward body _parallel_forward_view_batch
- * (parallel_beam.py:56-73; Triton body triton_parallel.py:598-745) driven
- * over the whole view range by Projectors.sparse_forward_project_view_range
- * (projectors.py:518-588). */
+ * `layout`.  `row_runs` (optional, 3 ints per image row: list position of
+ * the row's first listed pixel, its column, the run length; length 0 for
+ * rows without listed pixels) declares that every image row's pixels form
+ * one run of consecutive columns in list order; the ray-driven gather is
+ * then used, otherwise the binned gather.  Replaces the forward body
+ * _parallel_forward_view_batch (parallel_beam.py:56-73; Triton body
+ * triton_parallel.py:598-745) driven over the whole view range by
+ * Projectors.sparse_forward_project_view_range (projectors.py:518-588). */
 int mbir_par_forward(const float* values, const int64_t* pixel_indices,
-                     int32_t num_pixels, const float* view_table,
-                     const mbir_par_geom* geom, int32_t layout,
-                     float* sino_out, mbir_stream_t stream);
+                     int32_t num_pixels, const int32_t* row_runs,
+                     const float* view_table, const mbir_par_geom* geom,
+                     int32_t layout, float* sino_out, mbir_stream_t stream);
 
 /* Number of view splits mbir_par_back uses for num_pixels (>= 1). */
 int32_t mbir_par_back_splits(const mbir_par_geom* geom, int32_t num_pixels);
@@ -146,6 +180,47 @@ int mbir_par_fwd_update(const float* delta, const int64_t* pixel_indices,
                         const mbir_par_geom* geom, const float* alpha,
                         float* error_vcr, mbir_stream_t stream);
 
+/* ---- cone-beam projectors ---------------------------------------------- */
+
+/* Sparse cone forward projection (detector-side vertical fan, then the
+ * horizontal fan), written in `layout`.  Replaces _cone_forward_view_batch
+ * (cone_beam.py:161-234; Triton body triton_cone.py:640-746). */
+int mbir_cone_forward(const float* values, const int64_t* pixel_indices,
+                      int32_t num_pixels, const float* view_table,
+                      const mbir_cone_geom* geom, int32_t layout, float* sino_out,
+                      mbir_stream_t stream);
+
+/* Number of view splits mbir_cone_back uses for num_pixels (>= 1). */
+int32_t mbir_cone_back_splits(const mbir_cone_geom* geom, int32_t num_pixels);
+
+/* Sparse cone back projection of a channel-major sinogram (optionally times
+ * weights), weights raised to coeff_power, as num_splits partial sums over
+ * view ranges (num_splits, num_pixels, slices).  Replaces
+ * _cone_back_view_batch (cone_beam.py:237-299; Triton body
+ * triton_cone.py:333-462). */
+int mbir_cone_back(const float* sino_vcr, const float* weights_vcr,
+                   const int64_t* pixel_indices, int32_t num_pixels,
+                   const float* view_table, const mbir_cone_geom* geom,
+                   int32_t coeff_power, int32_t num_splits, float* partials,
+                   mbir_stream_t stream);
+
+/* Number of FP64 partial pairs the cone forward line search writes. */
+int32_t mbir_cone_fwd_partials_count(const mbir_cone_geom* geom);
+
+/* Cone forward of delta fused with the line-search reductions (see
+ * mbir_par_fwd_linesearch). */
+int mbir_cone_fwd_linesearch(const float* delta, const int64_t* pixel_indices,
+                             int32_t num_pixels, const float* view_table,
+                             const mbir_cone_geom* geom, const float* error_vcr,
+                             const float* weights_vcr, double* partials,
+                             mbir_stream_t stream);
+
+/* error -= alpha[0] * (A delta) for the cone operator. */
+int mbir_cone_fwd_update(const float* delta, const int64_t* pixel_indices,
+                         int32_t num_pixels, const float* view_table,
+                         const mbir_cone_geom* geom, const float* alpha,
+                         float* error_vcr, mbir_stream_t stream);
+
 /* ---- VCD subset step --------------------------------------------------- */
 
 /* Number of FP64 partial pairs mbir_vcd_direction writes. */
@@ -165,6 +240,21 @@ int mbir_vcd_direction(const float* flat_recon, const int64_t* pixel_indices,
                        int32_t num_splits, const float* fm_hessian,
                        const float* prox_input, const mbir_vcd_params* prm,
                        float* delta, double* partials, mbir_stream_t stream);
+
+/* Scratch floats mbir_vcd_direction_dc needs: 4 * num_pixels * num_slices. */
+int64_t mbir_vcd_direction_dc_scratch(int32_t num_pixels, int32_t num_slices);
+
+/* Cone DC-damped direction (cone_beam.py:48-55, 476-496):
+ * g = g_f + g_p, h_inv = 1/(H_f + H_p), gbar_k = sum_p h_inv*g / sum_p h_inv
+ * per slice over the subset (FP64, one workgroup per slice),
+ * delta = -(g - (1 - s_k) * gbar_k) * h_inv, with the prior line-term partials
+ * as mbir_vcd_direction writes them.  slice_profile holds s_k (float32). */
+int mbir_vcd_direction_dc(const float* flat_recon, const int64_t* pixel_indices,
+                          int32_t num_pixels, const float* back_parts,
+                          int32_t num_splits, const float* fm_hessian,
+                          const float* prox_input, const mbir_vcd_params* prm,
+                          const float* slice_profile, float* scratch, float* delta,
+                          double* partials, mbir_stream_t stream);
 
 /* alpha = clamp((fm*lin - prior_lin) / (fm*quad + prior_quad + eps32),
  * eps32, max_alpha), reduced in a fixed order; writes alpha[0] (float32)

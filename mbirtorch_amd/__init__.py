@@ -12,6 +12,7 @@ __version__ = '0.1.0'
 from ._lib import load as load_library  # noqa: F401
 from .autograd import (TorchProjector, back_project_differentiable,  # noqa: F401
                        forward_project_differentiable)
+from .cone_beam import ConeBeamModel, recon_simple_cone  # noqa: F401
 from .parallel_beam import ParallelBeamModel, recon_simple_parallel  # noqa: F401
 from .tomography_model import TomographyModel, dot_f64  # noqa: F401
 from .vcd_utils import (gen_full_indices, gen_partition_sequence,  # noqa: F401

@@ -34,6 +34,24 @@ class ParGeom(ctypes.Structure):
                 ('col_center', ctypes.c_float), ('pad', ctypes.c_float)]
 
 
+class ConeGeom(ctypes.Structure):
+    _fields_ = [('num_rows', ctypes.c_int32), ('num_cols', ctypes.c_int32),
+                ('num_slices', ctypes.c_int32), ('num_det_rows', ctypes.c_int32),
+                ('num_channels', ctypes.c_int32), ('num_views', ctypes.c_int32),
+                ('psf_radius', ctypes.c_int32), ('bp_psf_radius', ctypes.c_int32),
+                ('curved', ctypes.c_int32), ('unit_channel', ctypes.c_int32),
+                ('pad0', ctypes.c_int32), ('pad1', ctypes.c_int32),
+                ('delta_voxel', ctypes.c_float), ('delta_voxel_row', ctypes.c_float),
+                ('delta_voxel_slice', ctypes.c_float), ('delta_det_channel', ctypes.c_float),
+                ('delta_det_row', ctypes.c_float), ('det_channel_offset', ctypes.c_float),
+                ('det_row_offset', ctypes.c_float), ('source_detector_dist', ctypes.c_float),
+                ('source_iso_dist', ctypes.c_float), ('inv_magnification', ctypes.c_float),
+                ('det_center_channel', ctypes.c_float), ('det_center_row', ctypes.c_float),
+                ('row_center', ctypes.c_float), ('col_center', ctypes.c_float),
+                ('slice_center', ctypes.c_float), ('half_slice_span', ctypes.c_float),
+                ('area', ctypes.c_float), ('pad2', ctypes.c_float)]
+
+
 class VcdParams(ctypes.Structure):
     _fields_ = [('b', ctypes.c_float * 6), ('two_b', ctypes.c_float * 6),
                 ('sigma_x', ctypes.c_float), ('p', ctypes.c_float), ('q', ctypes.c_float),
@@ -51,18 +69,28 @@ _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
 _GP = ctypes.POINTER(ParGeom)
 _VP = ctypes.POINTER(VcdParams)
+_CP = ctypes.POINTER(ConeGeom)
 
 # name -> (restype, argtypes)
 SIGNATURES = {
     'mbir_version': (ctypes.c_int, []),
-    'mbir_par_forward': (ctypes.c_int, [_P, _P, _I32, _P, _GP, _I32, _P, _P]),
+    'mbir_par_forward': (ctypes.c_int, [_P, _P, _I32, _P, _P, _GP, _I32, _P, _P]),
     'mbir_par_back_splits': (_I32, [_GP, _I32]),
     'mbir_par_back': (ctypes.c_int, [_P, _P, _P, _I32, _P, _GP, _I32, _I32, _P, _P]),
     'mbir_reduce_splits': (ctypes.c_int, [_P, _I32, _I64, _P, _P]),
     'mbir_par_fwd_partials_count': (_I32, [_GP]),
     'mbir_par_fwd_linesearch': (ctypes.c_int, [_P, _P, _I32, _P, _GP, _P, _P, _P, _P]),
     'mbir_par_fwd_update': (ctypes.c_int, [_P, _P, _I32, _P, _GP, _P, _P, _P]),
+    'mbir_cone_forward': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _I32, _P, _P]),
+    'mbir_cone_back_splits': (_I32, [_CP, _I32]),
+    'mbir_cone_back': (ctypes.c_int, [_P, _P, _P, _I32, _P, _CP, _I32, _I32, _P, _P]),
+    'mbir_cone_fwd_partials_count': (_I32, [_CP]),
+    'mbir_cone_fwd_linesearch': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _P, _P, _P, _P]),
+    'mbir_cone_fwd_update': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _P, _P, _P]),
     'mbir_vcd_direction_partials_count': (_I32, [_I32, _I32]),
+    'mbir_vcd_direction_dc_scratch': (_I64, [_I32, _I32]),
+    'mbir_vcd_direction_dc': (ctypes.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _VP, _P, _P, _P,
+                                             _P, _P]),
     'mbir_vcd_direction': (ctypes.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _VP, _P, _P, _P]),
     'mbir_vcd_alpha': (ctypes.c_int, [_P, _I32, _P, _I32, _VP, _P, _P, _P]),
     'mbir_vcd_positivity': (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P]),

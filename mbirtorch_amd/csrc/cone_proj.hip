@@ -1,0 +1,541 @@
+// Cone-beam projectors for MI355X (gfx950): flat or curved detector,
+// circular or helical scans.
+//
+// Geometry for one (view, pixel), in the reference's float32 order
+// (cone_beam.py:59-158): rotated (x, y), per-pixel magnification
+// 1/(1/M - y/SDD), channel coordinate n_p (flat u = mag*x, curved
+// u = SDD*atan2(x, SID - y)), projected width W_c, weight scale, and the
+// vertical affine map m(k) = m0 + W_r*k from global slice k to detector row.
+// The cone-angle factor enters as 1/cos(phi) = sqrt(1 + (mag*z_k/SDD)^2)
+// (algebraically cos(atan2(mag*z_k, SDD))^-1).  Taps follow the reference's
+// windows: channels round(n_p) +- psf, back rows round(m(k)) +- psf, forward
+// slices round(k_m) +- bp_psf.
+//
+// Back (cone_beam.py:237-299): owner computes, atomic-free.  Geometry for a
+// (view batch x pixel set) goes to LDS once; lanes then map to slices, so the
+// taps of consecutive slices read consecutive detector rows of the
+// channel-major sinogram e[v, n, m].
+//
+// Forward (cone_beam.py:161-234): a workgroup owns one view and a band of RC
+// detector rows, with the (C x RC) tile in LDS; lanes map to rows, each row
+// gathers its voxel taps (detector-side vertical fan) and adds the
+// horizontal-fan taps into the tile with LDS float atomics.  Epilogues as in
+// the parallel projector: write, FP64 line search, or error update.
+#include "common.h"
+
+namespace mbir {
+
+constexpr int kCBlock = 256;
+constexpr int kCMaxTaps = 9;
+
+struct ConeView {
+  float cosv, sinv, angle, z_off;
+};
+
+struct ConePixel {
+  float n_p, Wc, ws, mag, m0, Wr;
+};
+
+__device__ __forceinline__ void cone_pixel(int64_t pix, const ConeView& vw,
+                                           const mbir_cone_geom& g, ConePixel& o) {
+  const int row = (int)(pix / g.num_cols);
+  const int col = (int)(pix - (int64_t)row * g.num_cols);
+  const float y_t = g.delta_voxel_row * ((float)row - g.row_center);
+  const float x_t = g.delta_voxel * ((float)col - g.col_center);
+  const float x = vw.cosv * x_t - vw.sinv * y_t;
+  const float y = vw.sinv * x_t + vw.cosv * y_t;
+  const float sdd = g.source_detector_dist;
+  const float mag = 1.0f / (g.inv_magnification - y / sdd);
+  float u, theta;
+  if (!g.curved) {
+    u = mag * x;
+    theta = atan2f(u, sdd);
+  } else {
+    u = sdd * atan2f(x, g.source_iso_dist - y);
+    theta = u / sdd;
+  }
+  float t = u + g.det_channel_offset;
+  if (!g.unit_channel) t = t / g.delta_det_channel;
+  o.n_p = t + g.det_center_channel;
+  const float d = vw.angle - theta;
+  const float fp = fmaxf(fabsf(cosf(d)) * g.delta_voxel, fabsf(sinf(d)) * g.delta_voxel_row);
+  float Wc = mag * (fp / g.delta_det_channel);
+  if (!g.curved) Wc = Wc / cosf(theta);
+  o.Wc = Wc;
+  o.ws = g.area / fp;
+  o.mag = mag;
+  o.Wr = (mag * g.delta_voxel_slice) / g.delta_det_row;
+  o.m0 = (mag * (vw.z_off - g.half_slice_span) + g.det_row_offset) / g.delta_det_row +
+         g.det_center_row;
+}
+
+// Trapezoid weight min(max((W+1)/2 - |d|, 0), min(1, W)).
+__device__ __forceinline__ float trap(float d, float W) {
+  return fminf(fmaxf((W + 1.0f) / 2.0f - fabsf(d), 0.0f), fminf(W, 1.0f));
+}
+
+__device__ __forceinline__ float inv_cos_phi(float mag, float z, float sdd) {
+  const float r = (mag * z) / sdd;
+  return sqrtf(1.0f + r * r);
+}
+
+// Horizontal taps: base = round(n_p) - psf, NH weights (zero off-detector).
+template <int NH>
+__device__ __forceinline__ void cone_htaps(const ConePixel& q, const mbir_cone_geom& g,
+                                           int& base, float (&a)[NH]) {
+  base = (int)rintf(q.n_p) - g.psf_radius;
+#pragma unroll
+  for (int k = 0; k < NH; ++k) {
+    const int n = base + k;
+    const float A = trap(q.n_p - (float)n, q.Wc) * q.ws;
+    a[k] = (n >= 0 && n < g.num_channels) ? A : 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Back projection.  LDS record per (view, pixel): base, NH weights, m0, Wr, mag.
+template <int RW, int NH, int NV, int CP, bool WEIGHTED>
+__global__ __launch_bounds__(kCBlock) void cone_back_kernel(
+    const float* __restrict__ sino, const float* __restrict__ wts,
+    const int64_t* __restrict__ pix, int n_pix, const ConeView* __restrict__ views,
+    mbir_cone_geom g, int views_per_split, float* __restrict__ out) {
+  constexpr int SLOTS = MBIR_WAVE / RW;
+  constexpr int ACC = RW >= 4 ? 4 : 2;
+  constexpr int PIX_WAVE = SLOTS * ACC;
+  constexpr int PIX_WG = (kCBlock / MBIR_WAVE) * PIX_WAVE;
+  constexpr int VB = PIX_WG >= 512 ? 1 : 512 / PIX_WG;
+  constexpr int REC = NH + 4;
+  __shared__ float s_rec[REC * VB * PIX_WG];
+  __shared__ float s_zoff[VB];
+
+  const int lane = threadIdx.x & (MBIR_WAVE - 1), wave = threadIdx.x / MBIR_WAVE;
+  const int slot = lane / RW, kl = lane % RW;
+  const int S = g.num_slices, C = g.num_channels, R = g.num_det_rows;
+  const int k = blockIdx.z * RW + kl;
+  const bool k_ok = k < S;
+  const float kf = (float)k;
+  const float zrel = g.delta_voxel_slice * (kf - g.slice_center);
+  const int p_base = blockIdx.x * PIX_WG;
+  const int v_begin = blockIdx.y * views_per_split;
+  const int v_end = min(v_begin + views_per_split, g.num_views);
+  constexpr int STRIDE = VB * PIX_WG;
+
+  float acc[ACC];
+#pragma unroll
+  for (int i = 0; i < ACC; ++i) acc[i] = 0.0f;
+
+  for (int vb0 = v_begin; vb0 < v_end; vb0 += VB) {
+    const int nvb = min(VB, v_end - vb0);
+    __syncthreads();
+    if (threadIdx.x < VB) s_zoff[threadIdx.x] = threadIdx.x < nvb ? views[vb0 + threadIdx.x].z_off : 0.0f;
+    for (int qd = threadIdx.x; qd < STRIDE; qd += kCBlock) {
+      const int j = qd % PIX_WG, b = qd / PIX_WG;
+      const int pg = p_base + j;
+      int base = 0;
+      float a[NH];
+      ConePixel q = {0, 1, 0, 1, -1e9f, 1};
+      if (b < nvb && pg < n_pix) {
+        cone_pixel(pix[pg], views[vb0 + b], g, q);
+        cone_htaps<NH>(q, g, base, a);
+      } else {
+#pragma unroll
+        for (int t = 0; t < NH; ++t) a[t] = 0.0f;
+      }
+      s_rec[qd] = __int_as_float(base);
+#pragma unroll
+      for (int t = 0; t < NH; ++t) s_rec[(1 + t) * STRIDE + qd] = CP == 2 ? a[t] * a[t] : a[t];
+      s_rec[(1 + NH) * STRIDE + qd] = q.m0;
+      s_rec[(2 + NH) * STRIDE + qd] = q.Wr;
+      s_rec[(3 + NH) * STRIDE + qd] = q.mag;
+    }
+    __syncthreads();
+    if (k_ok) {
+      for (int b = 0; b < nvb; ++b) {
+        const float z = zrel + s_zoff[b];
+        const size_t vofs = (size_t)(vb0 + b) * C * R;
+#pragma unroll
+        for (int i = 0; i < ACC; ++i) {
+          const int qd = b * PIX_WG + wave * PIX_WAVE + i * SLOTS + slot;
+          const int base = __float_as_int(s_rec[qd]);
+          const float m0 = s_rec[(1 + NH) * STRIDE + qd];
+          const float Wr = s_rec[(2 + NH) * STRIDE + qd];
+          const float mag = s_rec[(3 + NH) * STRIDE + qd];
+          const float ic = inv_cos_phi(mag, z, g.source_detector_dist);
+          const float m_p = m0 + Wr * kf;
+          const int mc = (int)rintf(m_p) - g.psf_radius;
+          float sum = 0.0f;
+#pragma unroll
+          for (int tv = 0; tv < NV; ++tv) {
+            const int mm = mc + tv;
+            float Av = fminf(fmaxf((Wr + 1.0f) / 2.0f - fabsf(m_p - (float)mm), 0.0f),
+                             fminf(Wr, 1.0f)) * ic;
+            Av = (mm >= 0 && mm < R) ? Av : 0.0f;
+            if (CP == 2) Av = Av * Av;
+            const int mcl = min(max(mm, 0), R - 1);
+            float det = 0.0f;
+#pragma unroll
+            for (int th = 0; th < NH; ++th) {
+              const float ah = s_rec[(1 + th) * STRIDE + qd];
+              const int n = min(max(base + th, 0), C - 1);
+              float e = sino[vofs + (size_t)n * R + mcl];
+              if (WEIGHTED) e = e * wts[vofs + (size_t)n * R + mcl];
+              det = fmaf(ah, e, det);
+            }
+            sum = fmaf(Av, det, sum);
+          }
+          acc[i] += sum;
+        }
+      }
+    }
+  }
+  if (k_ok) {
+#pragma unroll
+    for (int i = 0; i < ACC; ++i) {
+      const int pg = p_base + wave * PIX_WAVE + i * SLOTS + slot;
+      if (pg < n_pix) out[((size_t)blockIdx.y * n_pix + pg) * S + k] = acc[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward projection into a per-view row-band tile.
+enum ConeFwdMode { CF_WRITE_VCR = 0, CF_WRITE_VRC = 1, CF_LINESEARCH = 2, CF_UPDATE = 3 };
+
+template <int RC, int NH, int NK, int MODE, bool WEIGHTED>
+__global__ __launch_bounds__(kCBlock) void cone_fwd_tile_kernel(
+    const float* __restrict__ values, const int64_t* __restrict__ pix, int n_pix,
+    const ConeView* __restrict__ views, mbir_cone_geom g, float* __restrict__ sino,
+    const float* __restrict__ wts, const float* __restrict__ alpha,
+    double* __restrict__ partials) {
+  constexpr int TS = RC + 1;
+  constexpr int SLOTS = MBIR_WAVE / RC;
+  constexpr int REC = NH + 4;
+  extern __shared__ float smem[];
+  const int C = g.num_channels, R = g.num_det_rows, S = g.num_slices;
+  float* tile = smem;                      // C x TS
+  float* s_rec = tile + (size_t)C * TS;    // REC x kCBlock
+
+  const int v = blockIdx.x, m_lo = blockIdx.y * RC;
+  const int tid = threadIdx.x;
+  const int lane = tid & (MBIR_WAVE - 1), wave = tid / MBIR_WAVE;
+  const int slot = lane / RC, rl = lane % RC;
+  const int m = m_lo + rl;
+  const float mf = (float)m;
+  const ConeView vw = views[v];
+  const float sdd = g.source_detector_dist;
+
+  for (int i = tid; i < C * TS; i += kCBlock) tile[i] = 0.0f;
+
+  int64_t p_next = tid < n_pix ? pix[tid] : 0;
+  for (int pc = 0; pc < n_pix; pc += kCBlock) {
+    int base = 0;
+    float a[NH];
+    ConePixel q = {0, 1, 0, 1, -1e9f, 1};
+    const bool valid = pc + tid < n_pix;
+    if (valid) {
+      cone_pixel(p_next, vw, g, q);
+      cone_htaps<NH>(q, g, base, a);
+    } else {
+#pragma unroll
+      for (int t = 0; t < NH; ++t) a[t] = 0.0f;
+    }
+    __syncthreads();
+    s_rec[tid] = __int_as_float(base);
+#pragma unroll
+    for (int t = 0; t < NH; ++t) s_rec[(1 + t) * kCBlock + tid] = a[t];
+    s_rec[(1 + NH) * kCBlock + tid] = q.m0;
+    s_rec[(2 + NH) * kCBlock + tid] = q.Wr;
+    s_rec[(3 + NH) * kCBlock + tid] = q.mag;
+    __syncthreads();
+    if (pc + kCBlock + tid < n_pix) p_next = pix[pc + kCBlock + tid];
+    const int cnt = min(kCBlock, n_pix - pc);
+#pragma unroll 2
+    for (int jj = 0; jj < MBIR_WAVE; jj += SLOTS) {
+      const int j = wave * MBIR_WAVE + jj + slot;
+      const int jc = min(j, cnt - 1);
+      const float m0 = s_rec[(1 + NH) * kCBlock + j];
+      const float Wr = s_rec[(2 + NH) * kCBlock + j];
+      const float mag = s_rec[(3 + NH) * kCBlock + j];
+      // Detector row -> fractional slice (inverse of the affine map).
+      const float k_m = (mf - m0) / Wr;
+      const float kc = rintf(k_m);
+      const float m_p = Wr * (kc - k_m);
+      const float half = (Wr + 1.0f) / 2.0f, lmax = fminf(Wr, 1.0f);
+      const float* xp = values + (size_t)(pc + jc) * S;
+      float det = 0.0f;
+#pragma unroll
+      for (int tk = 0; tk < NK; ++tk) {
+        const int off = tk - (NK - 1) / 2;
+        const int ki = (int)kc + off;
+        float A = fminf(fmaxf(half - fabsf(m_p + Wr * (float)off), 0.0f), lmax);
+        const bool kv = ki >= 0 && ki < S && j < cnt && m < R;
+        A = kv ? A : 0.0f;
+        const int kcl = min(max(ki, 0), S - 1);
+        const float z = g.delta_voxel_slice * ((float)kcl - g.slice_center) + vw.z_off;
+        const float gv = xp[kcl] * inv_cos_phi(mag, z, sdd);
+        det = det + A * gv;
+      }
+      const int b = __float_as_int(s_rec[j]);
+#pragma unroll
+      for (int th = 0; th < NH; ++th) {
+        const int n = min(max(b + th, 0), C - 1);
+        atomicAdd(&tile[n * TS + rl], s_rec[(1 + th) * kCBlock + j] * det);
+      }
+    }
+  }
+  __syncthreads();
+
+  if (MODE == CF_WRITE_VRC) {
+    for (int i = tid; i < C * RC; i += kCBlock) {
+      const int rr = i / C, c = i - rr * C;
+      if (m_lo + rr < R) sino[((size_t)v * R + m_lo + rr) * C + c] = tile[c * TS + rr];
+    }
+  } else {
+    double lin = 0.0, quad = 0.0;
+    float al = 0.0f;
+    if (MODE == CF_UPDATE) al = alpha[0];
+    for (int i = tid; i < C * RC; i += kCBlock) {
+      const int c = i / RC, rr = i - c * RC;
+      if (m_lo + rr >= R) continue;
+      const size_t o = ((size_t)v * C + c) * R + m_lo + rr;
+      const float t = tile[c * TS + rr];
+      if (MODE == CF_WRITE_VCR) {
+        sino[o] = t;
+      } else if (MODE == CF_LINESEARCH) {
+        const float e = sino[o];
+        if (WEIGHTED) {
+          const float w = wts[o];
+          lin += (double)((w * e) * t);
+          quad += (double)((t * t) * w);
+        } else {
+          lin += (double)(e * t);
+          quad += (double)(t * t);
+        }
+      } else {
+        sino[o] = sino[o] - al * t;
+      }
+    }
+    if (MODE == CF_LINESEARCH) {
+      block_sum2_f64(lin, quad);
+      if (tid == 0) {
+        const size_t bi = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        partials[2 * bi] = lin;
+        partials[2 * bi + 1] = quad;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+static inline int cnext_pow2(int x) {
+  int p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+static int check_cone(const mbir_cone_geom* g) {
+  if (!g) return MBIR_ERR_ARG;
+  if (g->num_rows <= 0 || g->num_cols <= 0 || g->num_slices <= 0 || g->num_det_rows <= 0 ||
+      g->num_channels <= 0 || g->num_views <= 0)
+    return MBIR_ERR_SHAPE;
+  if (g->psf_radius < 1 || 2 * g->psf_radius + 1 > kCMaxTaps) return MBIR_ERR_GEOMETRY;
+  if (g->bp_psf_radius < 1 || 2 * g->bp_psf_radius + 1 > kCMaxTaps) return MBIR_ERR_GEOMETRY;
+  return MBIR_OK;
+}
+
+static inline int cone_rw(int S) { return S >= 64 ? 64 : cnext_pow2(S); }
+static inline int cone_pix_per_wg(int RW) {
+  return (kCBlock / MBIR_WAVE) * (MBIR_WAVE / RW) * (RW >= 4 ? 4 : 2);
+}
+
+static inline size_t cone_fwd_lds(int C, int rc, int nh) {
+  return ((size_t)C * (rc + 1) + (size_t)kCBlock * (nh + 4)) * 4;
+}
+
+static inline int cone_fwd_rc(const mbir_cone_geom* g) {
+  int rc = g->num_det_rows >= 64 ? 64 : cnext_pow2(g->num_det_rows);
+  const int nh = 2 * g->psf_radius + 1;
+  while (rc > 1 && cone_fwd_lds(g->num_channels, rc, nh) > 60 * 1024) rc >>= 1;
+  return rc;
+}
+
+template <int RW, int NH, int NV>
+static int cone_back_launch3(const float* sino, const float* w, const int64_t* pix, int P,
+                             const ConeView* views, const mbir_cone_geom& g, int cp, int ns,
+                             float* out, hipStream_t s) {
+  const int pwg = cone_pix_per_wg(RW);
+  const int vps = (g.num_views + ns - 1) / ns;
+  dim3 grid((P + pwg - 1) / pwg, ns, (g.num_slices + RW - 1) / RW);
+  if (cp == 1) {
+    if (w) cone_back_kernel<RW, NH, NV, 1, true><<<grid, kCBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
+    else cone_back_kernel<RW, NH, NV, 1, false><<<grid, kCBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
+  } else {
+    if (w) cone_back_kernel<RW, NH, NV, 2, true><<<grid, kCBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
+    else cone_back_kernel<RW, NH, NV, 2, false><<<grid, kCBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
+  }
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+template <int NT>
+static int cone_back_taps(const float* sino, const float* w, const int64_t* pix, int P,
+                          const ConeView* views, const mbir_cone_geom& g, int cp, int ns,
+                          float* out, hipStream_t s) {
+  switch (cone_rw(g.num_slices)) {
+    case 64: return cone_back_launch3<64, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
+    case 32: return cone_back_launch3<32, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
+    case 16: return cone_back_launch3<16, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
+    case 8: return cone_back_launch3<8, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
+    case 4: return cone_back_launch3<4, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
+    case 2: return cone_back_launch3<2, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
+    default: return cone_back_launch3<1, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
+  }
+}
+
+template <int RC, int NH, int NK>
+static int cone_fwd_launch3(int mode, const float* values, const int64_t* pix, int P,
+                            const ConeView* views, const mbir_cone_geom& g, float* sino,
+                            const float* w, const float* alpha, double* partials,
+                            hipStream_t s) {
+  dim3 grid(g.num_views, (g.num_det_rows + RC - 1) / RC);
+  const size_t lds = cone_fwd_lds(g.num_channels, RC, NH);
+  switch (mode) {
+    case CF_WRITE_VCR:
+      cone_fwd_tile_kernel<RC, NH, NK, CF_WRITE_VCR, false><<<grid, kCBlock, lds, s>>>(
+          values, pix, P, views, g, sino, w, alpha, partials);
+      break;
+    case CF_WRITE_VRC:
+      cone_fwd_tile_kernel<RC, NH, NK, CF_WRITE_VRC, false><<<grid, kCBlock, lds, s>>>(
+          values, pix, P, views, g, sino, w, alpha, partials);
+      break;
+    case CF_LINESEARCH:
+      if (w)
+        cone_fwd_tile_kernel<RC, NH, NK, CF_LINESEARCH, true><<<grid, kCBlock, lds, s>>>(
+            values, pix, P, views, g, sino, w, alpha, partials);
+      else
+        cone_fwd_tile_kernel<RC, NH, NK, CF_LINESEARCH, false><<<grid, kCBlock, lds, s>>>(
+            values, pix, P, views, g, sino, w, alpha, partials);
+      break;
+    default:
+      cone_fwd_tile_kernel<RC, NH, NK, CF_UPDATE, false><<<grid, kCBlock, lds, s>>>(
+          values, pix, P, views, g, sino, w, alpha, partials);
+  }
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+template <int NH, int NK>
+static int cone_fwd_rc_sw(int mode, const float* values, const int64_t* pix, int P,
+                          const ConeView* views, const mbir_cone_geom& g, float* sino,
+                          const float* w, const float* alpha, double* partials, hipStream_t s) {
+  switch (cone_fwd_rc(&g)) {
+    case 64: return cone_fwd_launch3<64, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    case 32: return cone_fwd_launch3<32, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    case 16: return cone_fwd_launch3<16, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    case 8: return cone_fwd_launch3<8, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    case 4: return cone_fwd_launch3<4, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    case 2: return cone_fwd_launch3<2, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    default: return cone_fwd_launch3<1, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+  }
+}
+
+template <int NH>
+static int cone_fwd_bp(int mode, const float* values, const int64_t* pix, int P,
+                       const ConeView* views, const mbir_cone_geom& g, float* sino,
+                       const float* w, const float* alpha, double* partials, hipStream_t s) {
+  switch (2 * g.bp_psf_radius + 1) {
+    case 3: return cone_fwd_rc_sw<NH, 3>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    case 5: return cone_fwd_rc_sw<NH, 5>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    case 7: return cone_fwd_rc_sw<NH, 7>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    default: return cone_fwd_rc_sw<NH, 9>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+  }
+}
+
+static int cone_fwd(int mode, const float* values, const int64_t* pix, int P,
+                    const ConeView* views, const mbir_cone_geom& g, float* sino, const float* w,
+                    const float* alpha, double* partials, hipStream_t s) {
+  switch (2 * g.psf_radius + 1) {
+    case 3: return cone_fwd_bp<3>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    case 5: return cone_fwd_bp<5>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    case 7: return cone_fwd_bp<7>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+    default: return cone_fwd_bp<9>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+  }
+}
+
+}  // namespace mbir
+
+using namespace mbir;
+
+extern "C" {
+
+int32_t mbir_cone_back_splits(const mbir_cone_geom* g, int32_t P) {
+  if (!g || P <= 0) return 1;
+  const int RW = cone_rw(g->num_slices);
+  const int pwg = cone_pix_per_wg(RW);
+  const long wg = (long)((P + pwg - 1) / pwg) * ((g->num_slices + RW - 1) / RW);
+  long ns = (2048 + wg - 1) / wg;
+  long max_ns = g->num_views / 8;
+  if (ns > max_ns) ns = max_ns;
+  if (ns < 1) ns = 1;
+  return (int32_t)ns;
+}
+
+int mbir_cone_back(const float* sino, const float* w, const int64_t* pix, int32_t P,
+                   const float* view_table, const mbir_cone_geom* g, int32_t cp, int32_t ns,
+                   float* out, mbir_stream_t stream) {
+  int st = check_cone(g);
+  if (st) return st;
+  if (!sino || !pix || !view_table || !out || P < 0) return MBIR_ERR_ARG;
+  if ((cp != 1 && cp != 2) || ns < 1 || ns > g->num_views) return MBIR_ERR_ARG;
+  if (P == 0) return MBIR_OK;
+  const ConeView* views = reinterpret_cast<const ConeView*>(view_table);
+  hipStream_t s = (hipStream_t)stream;
+  switch (2 * g->psf_radius + 1) {
+    case 3: return cone_back_taps<3>(sino, w, pix, P, views, *g, cp, ns, out, s);
+    case 5: return cone_back_taps<5>(sino, w, pix, P, views, *g, cp, ns, out, s);
+    case 7: return cone_back_taps<7>(sino, w, pix, P, views, *g, cp, ns, out, s);
+    default: return cone_back_taps<9>(sino, w, pix, P, views, *g, cp, ns, out, s);
+  }
+}
+
+int mbir_cone_forward(const float* values, const int64_t* pix, int32_t P,
+                      const float* view_table, const mbir_cone_geom* g, int32_t layout,
+                      float* out, mbir_stream_t stream) {
+  int st = check_cone(g);
+  if (st) return st;
+  if (!values || !pix || !view_table || !out || P < 0) return MBIR_ERR_ARG;
+  if (layout != MBIR_LAYOUT_VRC && layout != MBIR_LAYOUT_VCR) return MBIR_ERR_ARG;
+  return cone_fwd(layout == MBIR_LAYOUT_VRC ? CF_WRITE_VRC : CF_WRITE_VCR, values, pix, P,
+                  reinterpret_cast<const ConeView*>(view_table), *g, out, nullptr, nullptr,
+                  nullptr, (hipStream_t)stream);
+}
+
+int32_t mbir_cone_fwd_partials_count(const mbir_cone_geom* g) {
+  if (!g) return 0;
+  const int rc = cone_fwd_rc(g);
+  return g->num_views * ((g->num_det_rows + rc - 1) / rc);
+}
+
+int mbir_cone_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
+                             const float* view_table, const mbir_cone_geom* g,
+                             const float* error_vcr, const float* w, double* partials,
+                             mbir_stream_t stream) {
+  int st = check_cone(g);
+  if (st) return st;
+  if (!delta || !pix || !view_table || !error_vcr || !partials || P < 0) return MBIR_ERR_ARG;
+  return cone_fwd(CF_LINESEARCH, delta, pix, P, reinterpret_cast<const ConeView*>(view_table),
+                  *g, const_cast<float*>(error_vcr), w, nullptr, partials,
+                  (hipStream_t)stream);
+}
+
+int mbir_cone_fwd_update(const float* delta, const int64_t* pix, int32_t P,
+                         const float* view_table, const mbir_cone_geom* g, const float* alpha,
+                         float* error_vcr, mbir_stream_t stream) {
+  int st = check_cone(g);
+  if (st) return st;
+  if (!delta || !pix || !view_table || !error_vcr || !alpha || P < 0) return MBIR_ERR_ARG;
+  return cone_fwd(CF_UPDATE, delta, pix, P, reinterpret_cast<const ConeView*>(view_table), *g,
+                  error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -8,7 +8,9 @@
 // When every view has W <= 1 the nonzero taps are exactly floor(n_p) and
 // floor(n_p)+1 (|n - n_p| < (W+1)/2 <= 1), a subset of the reference's
 // round(n_p) +- psf taps with identical weights, so two taps are evaluated;
-// otherwise the reference's 2*psf+1 taps around round(n_p) are.
+// otherwise the reference's 2*psf+1 taps around round(n_p) are.  All three
+// kernels below evaluate n_p with one function (par_np), so a (view, pixel,
+// channel) weight is bit-identical in the forward and back projectors.
 //
 // Back projector (owner computes, atomic-free, deterministic): a workgroup
 // owns a set of voxel cylinders and a view range.  Geometry for
@@ -17,12 +19,20 @@
 // channel-major sinogram row e[v, n, 0:R].  Small pixel sets split the view
 // range over workgroups and leave partial sums reduced in a fixed order.
 //
-// Forward projector (per-view tile): a workgroup owns one view and a band of
-// RC detector rows and keeps the (C x RC) sinogram tile in LDS; every pixel
-// of the list adds its taps into the tile with LDS float atomics.  The tile
-// is then written out, or -- for the VCD subset step -- reduced against the
-// error sinogram (line search, FP64) or subtracted from it (update), so the
-// subset direction's projection never goes to HBM.
+// Forward projector, two gather forms (atomic-free, deterministic):
+//  * ray: for pixel lists in which every image row is one run of
+//    consecutive columns (the full region of reconstruction), each thread
+//    owns one detector channel of one view and a band of detector rows, walks
+//    the image rows (or columns, whichever the view's projection is steeper
+//    in), evaluates the 3-5 candidate pixels of the channel exactly, and
+//    accumulates in registers;
+//  * binned: for arbitrary pixel lists (the VCD subsets), a workgroup owns one
+//    view and 256 channels; chunks of the list are counting-sorted by channel
+//    in LDS, each bin ordered by list position, and every thread gathers its
+//    channel's bin.  Its epilogue writes the projection, or -- for the VCD
+//    subset step -- reduces it against the error sinogram (line search, FP64)
+//    or subtracts it from the error sinogram, so the subset direction's
+//    projection never goes to HBM.
 #include "common.h"
 
 namespace mbir {
@@ -30,33 +40,53 @@ namespace mbir {
 constexpr int kBlock = 256;
 constexpr int kMaxTaps = 9;  // psf_radius <= 4
 
+// Channel coordinate n_p of pixel (row, col) at one view.
+__device__ __forceinline__ float par_np(int row, int col, const ParView& vw,
+                                        const mbir_par_geom& g) {
+  const float y_t = g.delta_voxel_row * ((float)row - g.row_center);
+  const float x_t = g.delta_voxel * ((float)col - g.col_center);
+  const float x = vw.cosv * x_t - vw.sinv * y_t;
+  float t = x + g.det_channel_offset;
+  if (!g.unit_channel) t = t / g.delta_det_channel;
+  return t + g.det_center;
+}
+
+template <int NT>
+__device__ __forceinline__ int par_base(float n_p, const mbir_par_geom& g) {
+  return NT == 2 ? (int)floorf(n_p) : (int)rintf(n_p) - g.psf_radius;
+}
+
+__device__ __forceinline__ float par_weight(float n_p, int n, const ParView& vw) {
+  const float A = fmaxf(vw.halfw - fabsf(n_p - (float)n), 0.0f);
+  return fminf(A, vw.lmax) * vw.ws;
+}
+
 // Geometry for one (view, pixel): tap base index and NT weights (zero for
-// taps off the detector, base clamped so every tap index is in range when
-// the weight is nonzero).
+// taps off the detector).
 template <int NT>
 __device__ __forceinline__ void par_taps(int64_t pix, const ParView& vw,
                                          const mbir_par_geom& g, int& base,
                                          float (&a)[NT]) {
   const int row = (int)(pix / g.num_cols);
   const int col = (int)(pix - (int64_t)row * g.num_cols);
-  const float y_t = g.delta_voxel_row * ((float)row - g.row_center);
-  const float x_t = g.delta_voxel * ((float)col - g.col_center);
-  const float x = vw.cosv * x_t - vw.sinv * y_t;
-  float t = x + g.det_channel_offset;
-  if (!g.unit_channel) t = t / g.delta_det_channel;
-  const float n_p = t + g.det_center;
-  if (NT == 2) {
-    base = (int)floorf(n_p);
-  } else {
-    base = (int)rintf(n_p) - g.psf_radius;
-  }
+  const float n_p = par_np(row, col, vw, g);
+  base = par_base<NT>(n_p, g);
 #pragma unroll
   for (int k = 0; k < NT; ++k) {
     const int n = base + k;
-    float A = fmaxf(vw.halfw - fabsf(n_p - (float)n), 0.0f);
-    A = fminf(A, vw.lmax) * vw.ws;
-    a[k] = (n >= 0 && n < g.num_channels) ? A : 0.0f;
+    a[k] = (n >= 0 && n < g.num_channels) ? par_weight(n_p, n, vw) : 0.0f;
   }
+}
+
+// Weight of channel n for pixel (row, col) when n is one of the pixel's taps.
+template <int NT>
+__device__ __forceinline__ bool par_tap_of(int row, int col, int n, const ParView& vw,
+                                           const mbir_par_geom& g, float& A) {
+  const float n_p = par_np(row, col, vw, g);
+  const int k = n - par_base<NT>(n_p, g);
+  if (k < 0 || k >= NT) return false;
+  A = par_weight(n_p, n, vw);
+  return A != 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -147,104 +177,251 @@ __global__ void reduce_splits_kernel(const float* __restrict__ parts, int ns,
 }
 
 // ---------------------------------------------------------------------------
-// Forward projection into a per-view LDS tile.
+// Shared forward helpers: accumulate RPT rows of one voxel cylinder.
+template <int RPT, bool VEC>
+__device__ __forceinline__ void acc_rows(float (&acc)[RPT], float A, const float* xp, int nr) {
+  if (VEC) {
+#pragma unroll
+    for (int k = 0; k < RPT; k += 4) {
+      if (k < nr) {
+        const float4 x4 = *reinterpret_cast<const float4*>(xp + k);
+        acc[k] = fmaf(A, x4.x, acc[k]);
+        acc[k + 1] = fmaf(A, x4.y, acc[k + 1]);
+        acc[k + 2] = fmaf(A, x4.z, acc[k + 2]);
+        acc[k + 3] = fmaf(A, x4.w, acc[k + 3]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+      if (k < nr) acc[k] = fmaf(A, xp[k], acc[k]);
+  }
+}
+
 enum FwdMode { FWD_WRITE_VCR = 0, FWD_WRITE_VRC = 1, FWD_LINESEARCH = 2, FWD_UPDATE = 3 };
 
-template <int RC, int NT, int MODE, bool WEIGHTED>
-__global__ __launch_bounds__(kBlock) void par_fwd_tile_kernel(
+// Epilogue for a thread owning sinogram entries (v, n, r0 .. r0+nr).
+template <int RPT, int MODE, bool WEIGHTED>
+__device__ __forceinline__ void fwd_epilogue(const float (&acc)[RPT], int v, int n, int r0, int nr,
+                                             const mbir_par_geom& g, float* __restrict__ sino,
+                                             const float* __restrict__ wts, float al,
+                                             double& lin, double& quad) {
+  const int R = g.num_slices, C = g.num_channels;
+  if (MODE == FWD_WRITE_VRC) {
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+      if (k < nr) sino[((size_t)v * R + r0 + k) * C + n] = acc[k];
+    return;
+  }
+  const size_t o = ((size_t)v * C + n) * R + r0;
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    if (k >= nr) continue;
+    const float t = acc[k];
+    if (MODE == FWD_WRITE_VCR) {
+      sino[o + k] = t;
+    } else if (MODE == FWD_LINESEARCH) {
+      const float e = sino[o + k];
+      if (WEIGHTED) {
+        const float w = wts[o + k];
+        lin += (double)((w * e) * t);
+        quad += (double)((t * t) * w);
+      } else {
+        lin += (double)(e * t);
+        quad += (double)(t * t);
+      }
+    } else {
+      sino[o + k] = sino[o + k] - al * t;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ray-driven forward for row-structured pixel lists.  runs[3*i .. 3*i+2] =
+// (list position of the run's first pixel, first column, run length) for
+// image row i; a length of 0 means the row holds no listed pixel.
+constexpr int kRayBlock = 128;
+
+template <int RPT, int NT, bool VEC, int MODE>
+__global__ __launch_bounds__(kRayBlock) void par_fwd_ray_kernel(
+    const float* __restrict__ values, const int* __restrict__ runs,
+    const ParView* __restrict__ views, mbir_par_geom g, float* __restrict__ sino) {
+  const int n = blockIdx.x * kRayBlock + threadIdx.x;
+  const int v = blockIdx.y, r0 = blockIdx.z * RPT;
+  const int R = g.num_slices, C = g.num_channels, Nr = g.num_rows, Nc = g.num_cols;
+  if (n >= C) return;  // no barriers in this kernel
+  const int nr = min(RPT, R - r0);
+  const ParView vw = views[v];
+  float acc[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) acc[k] = 0.0f;
+
+  // Candidate windows come from the inverse map in float32 with a half-pixel
+  // margin; every candidate is then checked with the exact forward chain.
+  const float x_target = ((float)n - g.det_center) * g.delta_det_channel - g.det_channel_offset;
+  const float cd = vw.cosv * g.delta_voxel, sd = vw.sinv * g.delta_voxel_row;
+  if (fabsf(cd) >= fabsf(sd)) {
+    const float hj = vw.halfw * g.delta_det_channel / fabsf(cd) + 0.5f;
+    for (int i = 0; i < Nr; ++i) {
+      const int start = runs[3 * i], jmin = runs[3 * i + 1], cnt = runs[3 * i + 2];
+      if (cnt <= 0) continue;
+      const float y_t = g.delta_voxel_row * ((float)i - g.row_center);
+      const float jstar = (x_target + vw.sinv * y_t) / cd + g.col_center;
+      const int jlo = max((int)floorf(jstar - hj), jmin);
+      const int jhi = min((int)ceilf(jstar + hj), jmin + cnt - 1);
+      for (int j = jlo; j <= jhi; ++j) {
+        float A;
+        if (par_tap_of<NT>(i, j, n, vw, g, A))
+          acc_rows<RPT, VEC>(acc, A, values + (size_t)(start + j - jmin) * R + r0, nr);
+      }
+    }
+  } else {
+    const float hi = vw.halfw * g.delta_det_channel / fabsf(sd) + 0.5f;
+    for (int j = 0; j < Nc; ++j) {
+      const float x_t = g.delta_voxel * ((float)j - g.col_center);
+      const float istar = (vw.cosv * x_t - x_target) / sd + g.row_center;
+      const int ilo = max((int)floorf(istar - hi), 0);
+      const int ihi = min((int)ceilf(istar + hi), Nr - 1);
+      for (int i = ilo; i <= ihi; ++i) {
+        const int start = runs[3 * i], jmin = runs[3 * i + 1], cnt = runs[3 * i + 2];
+        if (j < jmin || j >= jmin + cnt) continue;
+        float A;
+        if (par_tap_of<NT>(i, j, n, vw, g, A))
+          acc_rows<RPT, VEC>(acc, A, values + (size_t)(start + j - jmin) * R + r0, nr);
+      }
+    }
+  }
+  double lin = 0.0, quad = 0.0;
+  fwd_epilogue<RPT, MODE, false>(acc, v, n, r0, nr, g, sino, nullptr, 0.0f, lin, quad);
+}
+
+// ---------------------------------------------------------------------------
+// Binned forward for arbitrary pixel lists.  Workgroup = (view, 256 channels,
+// RPT rows); thread t owns channel c0 + t.
+template <int NT>
+struct BinCfg {
+  static constexpr int CH = NT <= 2 ? 1024 : (NT <= 3 ? 512 : 256);  // pixels per chunk
+  static constexpr int PPT = CH / kBlock;                             // pixels per thread
+  static constexpr int ENT = CH * NT;                                 // entry capacity
+};
+
+// Exclusive block scan of one int per thread (256 threads).
+__device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) s_tmp[wid] = x;
+  __syncthreads();
+  int prefix = 0;
+  for (int w = 0; w < wid; ++w) prefix += s_tmp[w];
+  return prefix + x - v;
+}
+
+template <int RPT, int NT, bool VEC, int MODE, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
     const float* __restrict__ values, const int64_t* __restrict__ pix, int n_pix,
     const ParView* __restrict__ views, mbir_par_geom g, float* __restrict__ sino,
     const float* __restrict__ wts, const float* __restrict__ alpha,
     double* __restrict__ partials) {
-  constexpr int TS = RC + 1;  // odd stride: conflict-free transposed reads
-  constexpr int SLOTS = MBIR_WAVE / RC;
-  extern __shared__ float smem[];
-  const int C = g.num_channels, R = g.num_slices;
-  float* tile = smem;
-  int* s_n0 = reinterpret_cast<int*>(tile + (size_t)C * TS);
-  float* s_a = reinterpret_cast<float*>(s_n0 + kBlock);
+  using Cfg = BinCfg<NT>;
+  __shared__ int s_cnt[kBlock + 1];
+  __shared__ int s_off[kBlock + 1];
+  __shared__ int s_tmp[4];
+  __shared__ int s_ej[Cfg::ENT];
+  __shared__ float s_ea[Cfg::ENT];
 
-  const int v = blockIdx.x, r0 = blockIdx.y * RC;
-  const int lane = threadIdx.x & (MBIR_WAVE - 1), wave = threadIdx.x / MBIR_WAVE;
-  const int slot = lane / RC, rl = lane % RC;
-  const bool r_ok = (r0 + rl) < R;
+  const int tid = threadIdx.x;
+  const int v = blockIdx.x, c0 = blockIdx.y * kBlock, r0 = blockIdx.z * RPT;
+  const int R = g.num_slices, C = g.num_channels;
+  const int n = c0 + tid;
+  const int nr = min(RPT, R - r0);
   const ParView vw = views[v];
+  float acc[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) acc[k] = 0.0f;
 
-  for (int i = threadIdx.x; i < C * TS; i += kBlock) tile[i] = 0.0f;
-
-  for (int pc = 0; pc < n_pix; pc += kBlock) {
+  for (int pc = 0; pc < n_pix; pc += Cfg::CH) {
+    // Geometry of this thread's pixels; taps landing in [c0, c0+256) count.
+    int bin[Cfg::PPT][NT];
+    float wgt[Cfg::PPT][NT];
+    s_cnt[tid] = 0;
     __syncthreads();
-    {
-      const int pg = pc + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < Cfg::PPT; ++q) {
+      const int j = pc + tid + q * kBlock;
       int base = 0;
       float a[NT];
-      if (pg < n_pix) {
-        par_taps<NT>(pix[pg], vw, g, base, a);
+      if (j < n_pix) {
+        par_taps<NT>(pix[j], vw, g, base, a);
       } else {
 #pragma unroll
-        for (int k = 0; k < NT; ++k) a[k] = 0.0f;
+        for (int t = 0; t < NT; ++t) a[t] = 0.0f;
       }
-      s_n0[threadIdx.x] = base;
 #pragma unroll
-      for (int k = 0; k < NT; ++k) s_a[k * kBlock + threadIdx.x] = a[k];
+      for (int t = 0; t < NT; ++t) {
+        const int b = base + t - c0;
+        const bool in = a[t] != 0.0f && b >= 0 && b < kBlock;
+        bin[q][t] = in ? b : -1;
+        wgt[q][t] = a[t];
+        if (in) atomicAdd(&s_cnt[b], 1);
+      }
     }
     __syncthreads();
-    const int cnt = min(kBlock, n_pix - pc);
-    if (r_ok) {
-      for (int jj = wave * MBIR_WAVE; jj < wave * MBIR_WAVE + MBIR_WAVE; jj += SLOTS) {
-        const int j = jj + slot;
-        if (j < cnt) {
-          const float xv = values[(size_t)(pc + j) * R + r0 + rl];
-          const int base = s_n0[j];
+    const int mine = s_cnt[tid];
+    const int off = block_exclusive_scan(mine, s_tmp);
+    s_off[tid] = off;
+    if (tid == kBlock - 1) s_off[kBlock] = off + mine;
+    s_cnt[tid] = off;  // scatter cursor
+    __syncthreads();
 #pragma unroll
-          for (int t = 0; t < NT; ++t) {
-            const float a = s_a[t * kBlock + j];
-            if (a != 0.0f) atomicAdd(&tile[(base + t) * TS + rl], a * xv);
-          }
+    for (int q = 0; q < Cfg::PPT; ++q) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int b = bin[q][t];
+        if (b >= 0) {
+          const int slot = atomicAdd(&s_cnt[b], 1);
+          s_ej[slot] = tid + q * kBlock;  // position within the chunk
+          s_ea[slot] = wgt[q][t];
         }
       }
     }
+    __syncthreads();
+    // Order this channel's bin by list position (bins are small for random
+    // subsets), then gather.
+    const int e0 = s_off[tid], e1 = s_off[tid + 1];
+    for (int e = e0 + 1; e < e1; ++e) {
+      const int kj = s_ej[e];
+      const float ka = s_ea[e];
+      int f = e - 1;
+      while (f >= e0 && s_ej[f] > kj) {
+        s_ej[f + 1] = s_ej[f];
+        s_ea[f + 1] = s_ea[f];
+        --f;
+      }
+      s_ej[f + 1] = kj;
+      s_ea[f + 1] = ka;
+    }
+    if (n < C) {
+      for (int e = e0; e < e1; ++e)
+        acc_rows<RPT, VEC>(acc, s_ea[e], values + (size_t)(pc + s_ej[e]) * R + r0, nr);
+    }
+    __syncthreads();
   }
-  __syncthreads();
 
-  if (MODE == FWD_WRITE_VRC) {
-    // out[v, r, c]: consecutive threads take consecutive channels.
-    for (int i = threadIdx.x; i < C * RC; i += kBlock) {
-      const int rr = i / C, c = i - rr * C;
-      if (r0 + rr < R) sino[((size_t)v * R + r0 + rr) * C + c] = tile[c * TS + rr];
-    }
-  } else {
-    double lin = 0.0, quad = 0.0;
-    float al = 0.0f;
-    if (MODE == FWD_UPDATE) al = alpha[0];
-    for (int i = threadIdx.x; i < C * RC; i += kBlock) {
-      const int c = i / RC, rr = i - c * RC;
-      if (r0 + rr >= R) continue;
-      const size_t o = ((size_t)v * C + c) * R + r0 + rr;
-      const float t = tile[c * TS + rr];
-      if (MODE == FWD_WRITE_VCR) {
-        sino[o] = t;
-      } else if (MODE == FWD_LINESEARCH) {
-        const float e = sino[o];
-        if (WEIGHTED) {
-          const float w = wts[o];
-          lin += (double)((w * e) * t);
-          quad += (double)((t * t) * w);
-        } else {
-          lin += (double)(e * t);
-          quad += (double)(t * t);
-        }
-      } else {  // FWD_UPDATE
-        sino[o] = sino[o] - al * t;
-      }
-    }
-    if (MODE == FWD_LINESEARCH) {
-      block_sum2_f64(lin, quad);
-      if (threadIdx.x == 0) {
-        const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-        partials[2 * b] = lin;
-        partials[2 * b + 1] = quad;
-      }
+  double lin = 0.0, quad = 0.0;
+  float al = 0.0f;
+  if (MODE == FWD_UPDATE) al = alpha[0];
+  if (n < C) fwd_epilogue<RPT, MODE, WEIGHTED>(acc, v, n, r0, nr, g, sino, wts, al, lin, quad);
+  if (MODE == FWD_LINESEARCH) {
+    block_sum2_f64(lin, quad);
+    if (tid == 0) {
+      const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+      partials[2 * b] = lin;
+      partials[2 * b + 1] = quad;
     }
   }
 }
@@ -263,7 +440,8 @@ static int check_geom(const mbir_par_geom* g) {
       g->num_channels <= 0 || g->num_views <= 0)
     return MBIR_ERR_SHAPE;
   if (g->psf_radius < 0 || 2 * g->psf_radius + 1 > kMaxTaps) return MBIR_ERR_GEOMETRY;
-  if (!(g->delta_det_channel > 0.0f)) return MBIR_ERR_GEOMETRY;
+  if (!(g->delta_det_channel > 0.0f) || !(g->delta_voxel > 0.0f) || !(g->delta_voxel_row > 0.0f))
+    return MBIR_ERR_GEOMETRY;
   return MBIR_OK;
 }
 
@@ -273,14 +451,13 @@ static inline int back_pix_per_wg(int RW) {
   return (kBlock / MBIR_WAVE) * (MBIR_WAVE / RW) * (RW >= 4 ? 8 : 4);
 }
 
-// Rows per forward tile: largest power of two <= 64 that covers R and keeps
-// the tile within 60 KiB of LDS.
-static inline int fwd_rc(const mbir_par_geom* g) {
-  int rc = g->num_slices >= 64 ? 64 : next_pow2(g->num_slices);
-  while (rc > 1 && (size_t)g->num_channels * (rc + 1) * 4 + kBlock * 4 * (1 + kMaxTaps) >
-                       60 * 1024)
-    rc >>= 1;
-  return rc;
+// Rows per forward thread: 64, or the smallest power of two >= 4 covering R.
+static inline int fwd_rpt(int R) {
+  if (R > 32) return 64;
+  if (R > 16) return 32;
+  if (R > 8) return 16;
+  if (R > 4) return 8;
+  return 4;
 }
 
 template <int RW, int NT>
@@ -316,65 +493,92 @@ static int launch_back_nt(const float* sino, const float* w, const int64_t* pix,
   }
 }
 
-template <int RC, int NT>
-static int launch_fwd_rc(int mode, const float* values, const int64_t* pix, int P,
-                         const ParView* views, const mbir_par_geom& g, float* sino,
-                         const float* w, const float* alpha, double* partials,
-                         hipStream_t s) {
-  dim3 grid(g.num_views, (g.num_slices + RC - 1) / RC);
-  const size_t lds = (size_t)g.num_channels * (RC + 1) * 4 + kBlock * 4 * (1 + NT);
-  switch (mode) {
-    case FWD_WRITE_VCR:
-      par_fwd_tile_kernel<RC, NT, FWD_WRITE_VCR, false><<<grid, kBlock, lds, s>>>(
-          values, pix, P, views, g, sino, w, alpha, partials);
-      break;
-    case FWD_WRITE_VRC:
-      par_fwd_tile_kernel<RC, NT, FWD_WRITE_VRC, false><<<grid, kBlock, lds, s>>>(
-          values, pix, P, views, g, sino, w, alpha, partials);
-      break;
-    case FWD_LINESEARCH:
-      if (w)
-        par_fwd_tile_kernel<RC, NT, FWD_LINESEARCH, true><<<grid, kBlock, lds, s>>>(
-            values, pix, P, views, g, sino, w, alpha, partials);
-      else
-        par_fwd_tile_kernel<RC, NT, FWD_LINESEARCH, false><<<grid, kBlock, lds, s>>>(
-            values, pix, P, views, g, sino, w, alpha, partials);
-      break;
-    default:
-      par_fwd_tile_kernel<RC, NT, FWD_UPDATE, false><<<grid, kBlock, lds, s>>>(
-          values, pix, P, views, g, sino, w, alpha, partials);
-      break;
+// Forward dispatch: ray kernel when `runs` is given (write modes), binned
+// kernel otherwise.
+struct FwdArgs {
+  int mode;
+  const float* values;
+  const int64_t* pix;
+  int P;
+  const int* runs;
+  const ParView* views;
+  mbir_par_geom g;
+  float* sino;
+  const float* w;
+  const float* alpha;
+  double* partials;
+  hipStream_t s;
+};
+
+template <int RPT, int NT, bool VEC>
+static int launch_fwd3(const FwdArgs& a) {
+  const mbir_par_geom& g = a.g;
+  if (a.runs) {
+    dim3 grid((g.num_channels + kRayBlock - 1) / kRayBlock, g.num_views,
+              (g.num_slices + RPT - 1) / RPT);
+    if (a.mode == FWD_WRITE_VRC)
+      par_fwd_ray_kernel<RPT, NT, VEC, FWD_WRITE_VRC><<<grid, kRayBlock, 0, a.s>>>(
+          a.values, a.runs, a.views, g, a.sino);
+    else
+      par_fwd_ray_kernel<RPT, NT, VEC, FWD_WRITE_VCR><<<grid, kRayBlock, 0, a.s>>>(
+          a.values, a.runs, a.views, g, a.sino);
+  } else {
+    dim3 grid(g.num_views, (g.num_channels + kBlock - 1) / kBlock,
+              (g.num_slices + RPT - 1) / RPT);
+    switch (a.mode) {
+      case FWD_WRITE_VCR:
+        par_fwd_bin_kernel<RPT, NT, VEC, FWD_WRITE_VCR, false><<<grid, kBlock, 0, a.s>>>(
+            a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
+        break;
+      case FWD_WRITE_VRC:
+        par_fwd_bin_kernel<RPT, NT, VEC, FWD_WRITE_VRC, false><<<grid, kBlock, 0, a.s>>>(
+            a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
+        break;
+      case FWD_LINESEARCH:
+        if (a.w)
+          par_fwd_bin_kernel<RPT, NT, VEC, FWD_LINESEARCH, true><<<grid, kBlock, 0, a.s>>>(
+              a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
+        else
+          par_fwd_bin_kernel<RPT, NT, VEC, FWD_LINESEARCH, false><<<grid, kBlock, 0, a.s>>>(
+              a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
+        break;
+      default:
+        par_fwd_bin_kernel<RPT, NT, VEC, FWD_UPDATE, false><<<grid, kBlock, 0, a.s>>>(
+            a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
+    }
   }
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }
 
+template <int RPT, int NT>
+static int launch_fwd2(const FwdArgs& a) {
+  // float4 rows need R % 4 == 0 (row chunks start at multiples of RPT >= 4)
+  // and a 16-byte aligned value array.
+  if (a.g.num_slices % 4 == 0 && (reinterpret_cast<uintptr_t>(a.values) & 15) == 0)
+    return launch_fwd3<RPT, NT, true>(a);
+  return launch_fwd3<RPT, NT, false>(a);
+}
+
 template <int NT>
-static int launch_fwd_nt(int mode, const float* values, const int64_t* pix, int P,
-                         const ParView* views, const mbir_par_geom& g, float* sino,
-                         const float* w, const float* alpha, double* partials,
-                         hipStream_t s) {
-  switch (fwd_rc(&g)) {
-    case 64: return launch_fwd_rc<64, NT>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 32: return launch_fwd_rc<32, NT>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 16: return launch_fwd_rc<16, NT>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 8: return launch_fwd_rc<8, NT>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 4: return launch_fwd_rc<4, NT>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 2: return launch_fwd_rc<2, NT>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    default: return launch_fwd_rc<1, NT>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+static int launch_fwd1(const FwdArgs& a) {
+  switch (fwd_rpt(a.g.num_slices)) {
+    case 64: return launch_fwd2<64, NT>(a);
+    case 32: return launch_fwd2<32, NT>(a);
+    case 16: return launch_fwd2<16, NT>(a);
+    case 8: return launch_fwd2<8, NT>(a);
+    default: return launch_fwd2<4, NT>(a);
   }
 }
 
-static int launch_fwd(int mode, const float* values, const int64_t* pix, int P,
-                      const ParView* views, const mbir_par_geom& g, float* sino,
-                      const float* w, const float* alpha, double* partials, hipStream_t s) {
-  if (g.two_tap) return launch_fwd_nt<2>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-  switch (2 * g.psf_radius + 1) {
-    case 1: return launch_fwd_nt<1>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 3: return launch_fwd_nt<3>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 5: return launch_fwd_nt<5>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 7: return launch_fwd_nt<7>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    default: return launch_fwd_nt<9>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+static int launch_fwd(const FwdArgs& a) {
+  if (a.g.two_tap) return launch_fwd1<2>(a);
+  switch (2 * a.g.psf_radius + 1) {
+    case 1: return launch_fwd1<1>(a);
+    case 3: return launch_fwd1<3>(a);
+    case 5: return launch_fwd1<5>(a);
+    case 7: return launch_fwd1<7>(a);
+    default: return launch_fwd1<9>(a);
   }
 }
 
@@ -384,7 +588,7 @@ using namespace mbir;
 
 extern "C" {
 
-int mbir_version(void) { return 1; }
+int mbir_version(void) { return 2; }
 
 int32_t mbir_par_back_splits(const mbir_par_geom* g, int32_t P) {
   if (!g || P <= 0) return 1;
@@ -431,22 +635,24 @@ int mbir_reduce_splits(const float* parts, int32_t ns, int64_t n, float* out,
   return MBIR_OK;
 }
 
-int mbir_par_forward(const float* values, const int64_t* pix, int32_t P,
+int mbir_par_forward(const float* values, const int64_t* pix, int32_t P, const int32_t* runs,
                      const float* view_table, const mbir_par_geom* g, int32_t layout,
                      float* out, mbir_stream_t stream) {
   int st = check_geom(g);
   if (st) return st;
   if (!values || !pix || !view_table || !out || P < 0) return MBIR_ERR_ARG;
   if (layout != MBIR_LAYOUT_VRC && layout != MBIR_LAYOUT_VCR) return MBIR_ERR_ARG;
-  const int mode = layout == MBIR_LAYOUT_VRC ? FWD_WRITE_VRC : FWD_WRITE_VCR;
-  return launch_fwd(mode, values, pix, P, reinterpret_cast<const ParView*>(view_table), *g,
-                    out, nullptr, nullptr, nullptr, (hipStream_t)stream);
+  FwdArgs a{layout == MBIR_LAYOUT_VRC ? FWD_WRITE_VRC : FWD_WRITE_VCR,
+            values, pix, P, runs, reinterpret_cast<const ParView*>(view_table), *g, out,
+            nullptr, nullptr, nullptr, (hipStream_t)stream};
+  return launch_fwd(a);
 }
 
 int32_t mbir_par_fwd_partials_count(const mbir_par_geom* g) {
   if (!g) return 0;
-  const int rc = fwd_rc(g);
-  return g->num_views * ((g->num_slices + rc - 1) / rc);
+  const int rpt = fwd_rpt(g->num_slices);
+  return g->num_views * ((g->num_channels + kBlock - 1) / kBlock) *
+         ((g->num_slices + rpt - 1) / rpt);
 }
 
 int mbir_par_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
@@ -456,9 +662,10 @@ int mbir_par_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
   int st = check_geom(g);
   if (st) return st;
   if (!delta || !pix || !view_table || !error_vcr || !partials || P < 0) return MBIR_ERR_ARG;
-  return launch_fwd(FWD_LINESEARCH, delta, pix, P,
-                    reinterpret_cast<const ParView*>(view_table), *g,
-                    const_cast<float*>(error_vcr), w, nullptr, partials, (hipStream_t)stream);
+  FwdArgs a{FWD_LINESEARCH, delta, pix, P, nullptr,
+            reinterpret_cast<const ParView*>(view_table), *g, const_cast<float*>(error_vcr), w,
+            nullptr, partials, (hipStream_t)stream};
+  return launch_fwd(a);
 }
 
 int mbir_par_fwd_update(const float* delta, const int64_t* pix, int32_t P,
@@ -467,8 +674,9 @@ int mbir_par_fwd_update(const float* delta, const int64_t* pix, int32_t P,
   int st = check_geom(g);
   if (st) return st;
   if (!delta || !pix || !view_table || !error_vcr || !alpha || P < 0) return MBIR_ERR_ARG;
-  return launch_fwd(FWD_UPDATE, delta, pix, P, reinterpret_cast<const ParView*>(view_table),
-                    *g, error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream);
+  FwdArgs a{FWD_UPDATE, delta, pix, P, nullptr, reinterpret_cast<const ParView*>(view_table),
+            *g, error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream};
+  return launch_fwd(a);
 }
 
 }  // extern "C"

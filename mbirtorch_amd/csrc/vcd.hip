@@ -26,49 +26,119 @@ __device__ __forceinline__ float qg_b_tilde(float delta, const mbir_vcd_params& 
   return num / (den * den);
 }
 
+// Prior gradient/Hessian (qGGMRF surrogate or proximal prior) and forward
+// gradient/Hessian at subset element i = (p, s).
+__device__ __forceinline__ void vcd_terms(const float* __restrict__ x,
+                                          const int64_t* __restrict__ pix, int64_t i, int64_t total,
+                                          const float* __restrict__ back_parts, int nsplit,
+                                          const float* __restrict__ hess,
+                                          const float* __restrict__ prox,
+                                          const mbir_vcd_params& prm, float& fg, float& gp,
+                                          float& fh, float& hp) {
+  const int S = prm.num_slices, Nc = prm.num_cols, Nr = prm.num_rows;
+  const int p = (int)(i / S), s = (int)(i - (int64_t)p * S);
+  const int64_t pi = pix[p];
+  const float xs = x[pi * S + s];
+  if (prm.use_prox) {
+    gp = prm.inv_sigma_prox2 * (xs - prox[pi * S + s]);
+    hp = prm.inv_sigma_prox2;
+  } else {
+    // Slice neighbours: deltas v[s]-v[s-1] and v[s+1]-v[s], zero at the
+    // reflected ends (qggmrf.py:135-156).
+    const float dm = s > 0 ? xs - x[pi * S + s - 1] : 0.0f;
+    const float dp = s < S - 1 ? x[pi * S + s + 1] - xs : 0.0f;
+    const float cm = 2.0f * qg_b_tilde(dm, prm);
+    const float cp = 2.0f * qg_b_tilde(dp, prm);
+    gp = (-prm.b[4]) * (cp * dp) + prm.b[5] * (cm * dm);
+    hp = prm.b[4] * cp + prm.b[5] * cm;
+    // In-plane neighbours, indices clamped to the grid (qggmrf.py:158-181).
+    const int row = (int)(pi / Nc), col = (int)(pi - (int64_t)row * Nc);
+    const int nbr[4] = {min(row + 1, Nr - 1) * Nc + col, max(row - 1, 0) * Nc + col,
+                        row * Nc + min(col + 1, Nc - 1), row * Nc + max(col - 1, 0)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float d = xs - x[(int64_t)nbr[k] * S + s];
+      const float c2 = prm.two_b[k] * qg_b_tilde(d, prm);
+      gp = gp + c2 * d;
+      hp = hp + c2;
+    }
+  }
+  float bsum = back_parts[i];
+  for (int k = 1; k < nsplit; ++k) bsum += back_parts[(size_t)k * total + i];
+  fg = prm.neg_fm_constant * bsum;
+  fh = prm.fm_constant * hess[pi * S + s];
+}
+
 __global__ __launch_bounds__(kThreads) void vcd_direction_kernel(
     const float* __restrict__ x, const int64_t* __restrict__ pix, int P,
     const float* __restrict__ back_parts, int nsplit, const float* __restrict__ hess,
     const float* __restrict__ prox, mbir_vcd_params prm, float* __restrict__ delta,
     double* __restrict__ partials) {
-  const int S = prm.num_slices, Nc = prm.num_cols, Nr = prm.num_rows;
+  const int64_t total = (int64_t)P * prm.num_slices;
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  double pl = 0.0, pq = 0.0;
+  if (i < total) {
+    float fg, gp, fh, hp;
+    vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp);
+    const float d = -((fg + gp) / (fh + hp));
+    delta[i] = d;
+    pl = (double)(gp * d);
+    pq = (double)((hp * d) * d);
+  }
+  block_sum2_f64(pl, pq);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = pl;
+    partials[2 * blockIdx.x + 1] = pq;
+  }
+}
+
+// DC damping pass 1: g, h_inv and the prior terms per element.
+__global__ __launch_bounds__(kThreads) void vcd_dc_terms_kernel(
+    const float* __restrict__ x, const int64_t* __restrict__ pix, int P,
+    const float* __restrict__ back_parts, int nsplit, const float* __restrict__ hess,
+    const float* __restrict__ prox, mbir_vcd_params prm, float* __restrict__ scratch) {
+  const int64_t total = (int64_t)P * prm.num_slices;
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= total) return;
+  float fg, gp, fh, hp;
+  vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp);
+  scratch[i] = fg + gp;                  // g
+  scratch[total + i] = 1.0f / (fh + hp); // h_inv
+  scratch[2 * total + i] = gp;
+  scratch[3 * total + i] = hp;
+}
+
+// DC damping pass 2: one workgroup per slice, gbar_k in FP64 (fixed order),
+// then pass 3 forms delta.
+__global__ __launch_bounds__(kThreads) void vcd_dc_gbar_kernel(const float* __restrict__ scratch,
+                                                               int P, int S,
+                                                               double* __restrict__ gbar) {
+  const int s = blockIdx.x;
+  const int64_t total = (int64_t)P * S;
+  double num = 0.0, den = 0.0;
+  for (int p = threadIdx.x; p < P; p += kThreads) {
+    const size_t i = (size_t)p * S + s;
+    const float h = scratch[total + i];
+    num += (double)(h * scratch[i]);
+    den += (double)h;
+  }
+  block_sum2_f64(num, den);
+  if (threadIdx.x == 0) gbar[s] = num / den;
+}
+
+__global__ __launch_bounds__(kThreads) void vcd_dc_delta_kernel(
+    const float* __restrict__ scratch, const double* __restrict__ gbar,
+    const float* __restrict__ prof, int P, int S, float* __restrict__ delta,
+    double* __restrict__ partials) {
   const int64_t total = (int64_t)P * S;
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   double pl = 0.0, pq = 0.0;
   if (i < total) {
-    const int p = (int)(i / S), s = (int)(i - (int64_t)p * S);
-    const int64_t pi = pix[p];
-    const float xs = x[pi * S + s];
-    float gp, hp;
-    if (prm.use_prox) {
-      gp = prm.inv_sigma_prox2 * (xs - prox[pi * S + s]);
-      hp = prm.inv_sigma_prox2;
-    } else {
-      // Slice neighbours: deltas v[s]-v[s-1] and v[s+1]-v[s], zero at the
-      // reflected ends (qggmrf.py:135-156).
-      const float dm = s > 0 ? xs - x[pi * S + s - 1] : 0.0f;
-      const float dp = s < S - 1 ? x[pi * S + s + 1] - xs : 0.0f;
-      const float cm = 2.0f * qg_b_tilde(dm, prm);
-      const float cp = 2.0f * qg_b_tilde(dp, prm);
-      gp = (-prm.b[4]) * (cp * dp) + prm.b[5] * (cm * dm);
-      hp = prm.b[4] * cp + prm.b[5] * cm;
-      // In-plane neighbours, indices clamped to the grid (qggmrf.py:158-181).
-      const int row = (int)(pi / Nc), col = (int)(pi - (int64_t)row * Nc);
-      const int nbr[4] = {min(row + 1, Nr - 1) * Nc + col, max(row - 1, 0) * Nc + col,
-                          row * Nc + min(col + 1, Nc - 1), row * Nc + max(col - 1, 0)};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const float d = xs - x[(int64_t)nbr[k] * S + s];
-        const float c2 = prm.two_b[k] * qg_b_tilde(d, prm);
-        gp = gp + c2 * d;
-        hp = hp + c2;
-      }
-    }
-    float bsum = back_parts[i];
-    for (int k = 1; k < nsplit; ++k) bsum += back_parts[(size_t)k * total + i];
-    const float fg = prm.neg_fm_constant * bsum;
-    const float fh = prm.fm_constant * hess[pi * S + s];
-    const float d = -((fg + gp) / (fh + hp));
+    const int s = (int)(i % S);
+    const float g = scratch[i], h = scratch[total + i];
+    const float gp = scratch[2 * total + i], hp = scratch[3 * total + i];
+    const float shift = (1.0f - prof[s]) * (float)gbar[s];
+    const float d = -(g - shift) * h;
     delta[i] = d;
     pl = (double)(gp * d);
     pq = (double)((hp * d) * d);
@@ -243,6 +313,37 @@ int mbir_vcd_direction(const float* x, const int64_t* pix, int32_t P, const floa
   vcd_direction_kernel<<<nb, kThreads, 0, (hipStream_t)stream>>>(x, pix, P, back_parts, nsplit,
                                                                   hess, prox, *prm, delta,
                                                                   partials);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int64_t mbir_vcd_direction_dc_scratch(int32_t P, int32_t S) {
+  // 4 float arrays of P*S plus S doubles (as 2*S floats), rounded up.
+  return 4 * (int64_t)P * S + 2 * (int64_t)S + 4;
+}
+
+int mbir_vcd_direction_dc(const float* x, const int64_t* pix, int32_t P, const float* back_parts,
+                          int32_t nsplit, const float* hess, const float* prox,
+                          const mbir_vcd_params* prm, const float* prof, float* scratch,
+                          float* delta, double* partials, mbir_stream_t stream) {
+  if (!x || !pix || !back_parts || !hess || !prm || !prof || !scratch || !delta || !partials)
+    return MBIR_ERR_ARG;
+  if (prm->use_prox && !prox) return MBIR_ERR_ARG;
+  if (P <= 0 || nsplit < 1 || prm->num_slices <= 0) return MBIR_ERR_SHAPE;
+  hipStream_t s = (hipStream_t)stream;
+  const int S = prm->num_slices;
+  const int nb = mbir_vcd_direction_partials_count(P, S);
+  const int64_t total = (int64_t)P * S;
+  // gbar lives after the four float arrays, 8-byte aligned.
+  uintptr_t gaddr = reinterpret_cast<uintptr_t>(scratch + 4 * total);
+  gaddr = (gaddr + 7) & ~(uintptr_t)7;
+  double* gbar = reinterpret_cast<double*>(gaddr);
+  vcd_dc_terms_kernel<<<nb, kThreads, 0, s>>>(x, pix, P, back_parts, nsplit, hess, prox, *prm,
+                                               scratch);
+  MBIR_CHECK_LAUNCH();
+  vcd_dc_gbar_kernel<<<S, kThreads, 0, s>>>(scratch, P, S, gbar);
+  MBIR_CHECK_LAUNCH();
+  vcd_dc_delta_kernel<<<nb, kThreads, 0, s>>>(scratch, gbar, prof, P, S, delta, partials);
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }

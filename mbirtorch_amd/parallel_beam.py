@@ -29,6 +29,31 @@ def parallel_view_table(angles, delta_voxel, delta_voxel_row, delta_det_channel)
     return torch.stack([c, s, W, ws, halfw, lmax, z, z], dim=1).contiguous()
 
 
+def row_runs(idx, num_rows, num_cols):
+    """Image-row run table (int32 (num_rows, 3): list position of the first
+    listed pixel of the row, its column, run length) when every image row's
+    listed pixels are consecutive columns in increasing list order, else
+    None.  One device-to-host synchronisation."""
+    P = int(idx.shape[0])
+    if P == 0:
+        return None
+    rows = torch.div(idx, num_cols, rounding_mode='floor')
+    cols = idx - rows * num_cols
+    if P > 1:
+        d = idx[1:] - idx[:-1]
+        same = rows[1:] == rows[:-1]
+        ok = torch.all(d > 0) & torch.all(~same | (d == 1))
+    else:
+        ok = torch.ones((), dtype=torch.bool, device=idx.device)
+    ok = ok & (rows[0] >= 0) & (rows[-1] < num_rows)
+    if not bool(ok):
+        return None
+    counts = torch.bincount(rows, minlength=num_rows)
+    starts = torch.cumsum(counts, 0) - counts
+    jmin = torch.where(counts > 0, cols[starts.clamp(max=P - 1)], torch.zeros_like(counts))
+    return torch.stack([starts, jmin, counts], dim=1).to(torch.int32).contiguous()
+
+
 class ParallelProjector:
     """Device projector for fixed parallel-beam parameters."""
 
@@ -74,16 +99,23 @@ class ParallelProjector:
         if idx.dim() != 1 or idx.shape[0] != values.shape[0]:
             raise ValueError('pixel_indices must be 1-D with one entry per voxel cylinder')
 
-    def forward(self, values, idx, layout=_lib.MBIR_LAYOUT_VRC):
+    def forward(self, values, idx, layout=_lib.MBIR_LAYOUT_VRC, runs=None):
+        """Forward projection of cylinders ``values`` at ``idx``.  ``runs`` is
+        the image-row run table of ``idx`` (see row_runs); None derives it,
+        False forces the binned gather."""
         _lib.require_gpu(values)
         values = values.contiguous()
         self._check_values(values, idx)
+        if runs is None:
+            runs = row_runs(idx, self.geom.num_rows, self.geom.num_cols)
         V, R, C = self.shape
         shape = (V, R, C) if layout == _lib.MBIR_LAYOUT_VRC else (V, C, R)
         out = torch.empty(shape, dtype=torch.float32, device=values.device)
         _lib.check(_lib.lib().mbir_par_forward(
-            _lib.ptr(values), _lib.ptr(idx), int(idx.shape[0]), _lib.ptr(self.table),
-            self.geom, layout, _lib.ptr(out), _lib.stream_ptr()), 'parallel forward')
+            _lib.ptr(values), _lib.ptr(idx), int(idx.shape[0]),
+            _lib.ptr(runs) if runs is not False and runs is not None else None,
+            _lib.ptr(self.table), self.geom, layout, _lib.ptr(out), _lib.stream_ptr()),
+            'parallel forward')
         return out
 
     def back_into(self, sino_vcr, idx, parts, ns, weights=None):

@@ -201,12 +201,27 @@ class TomographyModel(ParameterHandler):
         return self.projector().back(self._vrc_to_vcr(sino), idx.contiguous(),
                                      coeff_power=coeff_power)
 
+    def full_index_runs_device(self):
+        """Image-row run table of the full index list (cached with it)."""
+        idx = self.full_indices_device()
+        cache = getattr(self, '_runs_cache', None)
+        if cache is None or cache[0] is not idx:
+            from .parallel_beam import row_runs
+            rs = self.get_params('recon_shape')
+            self._runs_cache = (idx, row_runs(idx, rs[0], rs[1]))
+        return self._runs_cache[1]
+
     def forward_project(self, recon, output_sharded=False):
         recon = self._recon_to_device(recon)
         idx = self.full_indices_device()
         values = recon.reshape(-1, recon.shape[-1])[idx].contiguous()
-        sino = self.projector().forward(values, idx, layout=_lib.MBIR_LAYOUT_VRC)
+        sino = self.projector().forward(values, idx, layout=_lib.MBIR_LAYOUT_VRC,
+                                        runs=self._full_runs_or_false())
         return sino if output_sharded else self._to_host(sino)
+
+    def _full_runs_or_false(self):
+        runs = self.full_index_runs_device()
+        return False if runs is None else runs
 
     def _scatter_flat(self, cyl, idx):
         recon_shape = self.get_params('recon_shape')
@@ -280,6 +295,11 @@ class TomographyModel(ParameterHandler):
     def recon_direct(self, sinogram, filter_name='ramp', output_sharded=False):
         """Direct (filtered back projection) reconstruction; geometry specific."""
         raise NotImplementedError
+
+    def _direction_profile(self):
+        """Per-slice damping profile for the VCD direction (None: the plain
+        preconditioned gradient, tomography_model.py:2420-2432)."""
+        return None
 
     def direct_recon(self, sinogram, filter_name='ramp', output_sharded=False):
         """Alias of :meth:`recon_direct`."""
@@ -420,7 +440,8 @@ class TomographyModel(ParameterHandler):
         (tomography_model.py:1666-1712); inner products in FP64."""
         idx = self.full_indices_device()
         fwd = self.projector().forward(x_flat[idx].contiguous(), idx,
-                                       layout=_lib.MBIR_LAYOUT_VCR)
+                                       layout=_lib.MBIR_LAYOUT_VCR,
+                                       runs=self._full_runs_or_false())
         wf = fwd if w_vcr is None else w_vcr * fwd
         nrm = dot_f64(wf, fwd)
         alpha = dot_f64(wf, y_vcr) / nrm if (nrm > 0 and scale) else 1
@@ -600,21 +621,43 @@ class TomographyModel(ParameterHandler):
                  do_initialization=True, stop_threshold_change_pct=0.2, max_iterations=3,
                  first_iteration=0, logfile_path='~/.mbirtorch/logs/prox.log', print_logs=True,
                  output_sharded=False):
-        """Proximal map for Plug-and-Play (tomography_model.py:3251-3372)."""
-        if sigma_prox is not None:
-            self.set_params(no_warning=True, sigma_prox=sigma_prox)
+        """Proximal map for Plug-and-Play (tomography_model.py:3251-3372): VCD
+        with the prior (x - prox_input)^2 / (2 sigma_prox^2).  Initialization
+        (partitions, auto-regularization) is cached across calls unless
+        do_initialization; an explicit sigma_prox overrides the automatic one
+        for this call only.  Returns (recon, recon_dict)."""
         if do_initialization or self.prox_data is None:
-            data = self.initialize_recon(sinogram, weights, init_recon, max_iterations,
-                                         first_iteration, logfile_path=logfile_path,
-                                         print_logs=print_logs)
-            self.prox_data = data
-        (sinogram, weights, _, partitions, seq, granularity, reg) = self.prox_data
-        with torch.no_grad():
-            recon, losses = self._vcd_recon(sinogram, partitions, seq,
-                                            stop_threshold_change_pct, weights=weights,
-                                            init_recon=init_recon, prox_input=prox_input,
-                                            first_iteration=first_iteration)
-        return recon if output_sharded else self._to_host(recon)
+            (sinogram, weights, init_recon, partitions, seq, granularity,
+             reg) = self.initialize_recon(sinogram, weights, init_recon, max_iterations,
+                                          first_iteration, logfile_path=logfile_path,
+                                          print_logs=print_logs)
+            self.prox_data = (partitions, seq, granularity, reg)
+        else:
+            partitions, seq, granularity, reg = self.prox_data
+        saved_sigma_prox = self.get_params('sigma_prox')
+        if sigma_prox is not None:
+            reg = dict(reg, sigma_prox=sigma_prox)
+            self.set_params(no_warning=True, sigma_prox=sigma_prox,
+                            auto_regularize_flag=self.get_params('auto_regularize_flag'))
+        try:
+            with torch.no_grad():
+                recon, losses = self._vcd_recon(sinogram, partitions, seq,
+                                                stop_threshold_change_pct, weights=weights,
+                                                init_recon=init_recon, prox_input=prox_input,
+                                                first_iteration=first_iteration)
+        finally:
+            self.set_params(no_warning=True, sigma_prox=saved_sigma_prox)
+        recon_params = dict(zip(RECON_PARAM_NAMES, [
+            len(losses[0]), granularity, [int(v) for v in seq],
+            [float(v) for v in losses[0]], [0], reg,
+            [100 * float(v) for v in losses[2]], [float(v) for v in losses[3]],
+            [[float(v) for v in row] for row in losses[4]]]))
+        for h in list(self.logger.handlers):
+            h.flush()
+        self.close_log_file()
+        notes = 'Proximal map completed: {}\n\n'.format(datetime.datetime.now())
+        recon_dict = self.get_recon_dict(recon_params, notes=notes)
+        return (recon if output_sharded else self._to_host(recon)), recon_dict
 
     def get_recon_dict(self, recon_params=None, notes=None, save_log=True, save_model=True):
         d = {'recon_params': recon_params if recon_params is not None
@@ -660,7 +703,11 @@ class _VcdEngine:
         self.stats = torch.zeros(self.S + 6, dtype=torch.float64, device=dev)
         self.reducer = _Reducer(dev)
         self.positivity = bool(model.get_params('positivity_flag'))
-        self.w_mean = None
+        self.profile = model._direction_profile()
+        self.dc_scratch = None
+        if self.profile is not None:
+            n = int(_lib.lib().mbir_vcd_direction_dc_scratch(max_p, self.S))
+            self.dc_scratch = torch.empty(n, dtype=torch.float32, device=dev)
 
     def begin_iteration(self):
         self.stats.zero_()
@@ -673,10 +720,17 @@ class _VcdEngine:
         parts = self.back_parts[:ns * P * self.S]
         self.ops.back_into(self.e, idx, parts, ns, weights=self.w)
         n_dir = int(lib.mbir_vcd_direction_partials_count(P, self.S))
-        _lib.check(lib.mbir_vcd_direction(
-            _lib.ptr(self.x), _lib.ptr(idx), P, _lib.ptr(parts), ns, _lib.ptr(self.hess),
-            _lib.ptr(self.prox), self.prm, _lib.ptr(self.delta), _lib.ptr(self.dir_parts), st),
-            'vcd direction')
+        if self.profile is None:
+            _lib.check(lib.mbir_vcd_direction(
+                _lib.ptr(self.x), _lib.ptr(idx), P, _lib.ptr(parts), ns, _lib.ptr(self.hess),
+                _lib.ptr(self.prox), self.prm, _lib.ptr(self.delta), _lib.ptr(self.dir_parts),
+                st), 'vcd direction')
+        else:
+            _lib.check(lib.mbir_vcd_direction_dc(
+                _lib.ptr(self.x), _lib.ptr(idx), P, _lib.ptr(parts), ns, _lib.ptr(self.hess),
+                _lib.ptr(self.prox), self.prm, _lib.ptr(self.profile),
+                _lib.ptr(self.dc_scratch), _lib.ptr(self.delta), _lib.ptr(self.dir_parts), st),
+                'vcd dc direction')
         self.ops.fwd_linesearch(self.delta, idx, self.e, self.w, self.fwd_parts)
         _lib.check(lib.mbir_vcd_alpha(_lib.ptr(self.dir_parts), n_dir, _lib.ptr(self.fwd_parts),
                                       self.n_fwd, self.prm, _lib.ptr(self.alpha),

@@ -140,10 +140,11 @@ def auto_regularization(sino, weights, recon_shape, delta_voxel,
     return float(sigma_y), float(sigma_x), float(sigma_x)
 
 
-# -- the operator ---------------------------------------------------------------
+# -- the operators ---------------------------------------------------------------
 class ParallelSystem:
     """Parallel-beam operator for the VCD oracle: the full ROR system matrix
-    A (V*C, P) in FP64, sinogram in (V, R, C)."""
+    A (V*C, P) in FP64 acting on each slice; sinogram vectors are the
+    (V*C, R) matrix of a (V, R, C) sinogram, flattened."""
 
     def __init__(self, angles, geom, recon_shape, use_ror_mask=True):
         self.angles = np.asarray(angles, dtype=np.float32)
@@ -158,6 +159,7 @@ class ParallelSystem:
         self.A2 = _par.system_matrix(self.pix, self.angles, geom, psf,
                                      coeff_power=2).tocsc()
         self.V, self.C = len(angles), geom['num_channels']
+        self.S = recon_shape[2]
         self.magnification = 1.0
 
     def cols(self, idx):
@@ -166,26 +168,94 @@ class ParallelSystem:
             raise ValueError('index outside the ROR')
         return p
 
-    # sinogram (V, R, C) <-> matrix (V*C, R)
-    def to_mat(self, s):
+    def to_vec(self, s):
         V, R, C = s.shape
-        return np.asarray(s, dtype=np.float64).transpose(0, 2, 1).reshape(V * C, R)
+        return np.asarray(s, dtype=np.float64).transpose(0, 2, 1).reshape(-1)
 
-    def from_mat(self, m):
-        R = m.shape[1]
-        return m.reshape(self.V, self.C, R).transpose(0, 2, 1)
+    def from_vec(self, v):
+        return v.reshape(self.V, self.C, self.S).transpose(0, 2, 1)
+
+    def fwd(self, idx, d):
+        return np.asarray(self.A[:, self.cols(idx)] @ d).reshape(-1)
+
+    def back(self, idx, ev):
+        return np.asarray(self.A[:, self.cols(idx)].T @ ev.reshape(-1, self.S))
+
+    def hessian(self, wv):
+        h = np.zeros((self.recon_shape[0] * self.recon_shape[1], self.S))
+        h[self.pix] = self.A2.T @ wv.reshape(-1, self.S)
+        return h
 
     def direct_recon(self, sino):
         """FBP: filter, then the adjoint at the ROR pixels -- parallel_beam.py:321-368."""
         f = _filters.fbp_filter_parallel(sino, self.geom['delta_voxel'],
                                          self.geom['delta_voxel_row'])
-        cyl = self.A.T @ self.to_mat(f)
-        flat = np.zeros((self.recon_shape[0] * self.recon_shape[1], cyl.shape[1]))
-        flat[self.pix] = cyl
+        flat = np.zeros((self.recon_shape[0] * self.recon_shape[1], self.S))
+        flat[self.pix] = self.back(self.pix, self.to_vec(f))
         return flat
 
     def direction(self, fg, pg, fh, ph):
         return -((fg + pg) / (fh + ph))
+
+
+class ConeSystem:
+    """Cone-beam operator for the VCD oracle: A (V*R*C, P*S) from
+    oracle/cone.py, FDK initialisation and the DC-damped direction
+    (cone_beam.py:48-55, 476-496)."""
+
+    def __init__(self, g, use_ror_mask=True, dc_damping=True):
+        from . import cone as _cone
+        self._cone = _cone
+        self.g = g
+        self.recon_shape = tuple(g['recon_shape'])
+        self.S = self.recon_shape[2]
+        self.pix = _par.full_indices(self.recon_shape, use_ror_mask)
+        self.pos = -np.ones(self.recon_shape[0] * self.recon_shape[1], dtype=np.int64)
+        self.pos[self.pix] = np.arange(len(self.pix))
+        self.A = _cone.system_matrix(self.pix, g).tocsc()
+        self.A2 = _cone.system_matrix(self.pix, g, coeff_power=2).tocsc()
+        self.geom = dict(delta_voxel=g['dv'], delta_det_channel=g['ddc'])
+        self.magnification = g['M']
+        self.s_prof = _cone.dc_damping_profile(g) if dc_damping else None
+
+    def _cols(self, idx):
+        p = self.pos[np.asarray(idx)]
+        if np.any(p < 0):
+            raise ValueError('index outside the ROR')
+        return (p[:, None] * self.S + np.arange(self.S)[None, :]).reshape(-1)
+
+    def to_vec(self, s):
+        return np.asarray(s, dtype=np.float64).reshape(-1)
+
+    def from_vec(self, v):
+        return v.reshape(self.g['V'], self.g['R'], self.g['C'])
+
+    def fwd(self, idx, d):
+        return np.asarray(self.A[:, self._cols(idx)] @ np.asarray(d).reshape(-1))
+
+    def back(self, idx, ev):
+        return np.asarray(self.A[:, self._cols(idx)].T @ ev).reshape(len(idx), self.S)
+
+    def hessian(self, wv):
+        h = np.zeros((self.recon_shape[0] * self.recon_shape[1], self.S))
+        h[self.pix] = (self.A2.T @ wv).reshape(-1, self.S)
+        return h
+
+    def direct_recon(self, sino):
+        c = self._cone
+        g = self.g
+        taps = _filters.ramp_filter(g['C']) * c.fdk_scale(g) * (np.pi / g['V'])
+        f = _filters.apply_row_filter(sino, taps, row_weight=c.fdk_row_weight(g))
+        flat = np.zeros((self.recon_shape[0] * self.recon_shape[1], self.S))
+        flat[self.pix] = self.back(self.pix, self.to_vec(f))
+        if float(np.max(g['z_shifts']) - np.min(g['z_shifts'])) > 0:
+            flat = flat * c.helical_z_weight(g)[None, :]
+        return flat
+
+    def direction(self, fg, pg, fh, ph):
+        if self.s_prof is None:
+            return -((fg + pg) / (fh + ph))
+        return self._cone.dc_damped_direction(fg, pg, fh, ph, self.s_prof)
 
 
 def vcd_recon(system, sino, params, max_iterations, weights=None,
@@ -207,12 +277,12 @@ def vcd_recon(system, sino, params, max_iterations, weights=None,
             system.geom['delta_det_channel'], system.magnification,
             prm['snr_db'], prm['sharpness'])
         prm.update(sigma_y=sy, sigma_x=sx, sigma_prox=sp)
-    y = system.to_mat(sino)
-    w = None if weights is None else system.to_mat(weights)
+    y = system.to_vec(sino)
+    w = None if weights is None else system.to_vec(weights)
     # Initial state (tomography_model.py:2803-2833, 1666-1712).
     if init_recon is None:
         x = system.direct_recon(sino)
-        fwd = system.A @ x[system.pix]
+        fwd = system.fwd(system.pix, x[system.pix])
         wf = fwd if w is None else w * fwd
         nrm = np.sum(wf * fwd)
         alpha = np.sum(wf * y) / nrm if nrm > 0 else 1.0
@@ -220,11 +290,9 @@ def vcd_recon(system, sino, params, max_iterations, weights=None,
         x = alpha * x
     else:
         x = np.asarray(init_recon, dtype=np.float64).reshape(-1, S).copy()
-        e = y - system.A @ x[system.pix]
+        e = y - system.fwd(system.pix, x[system.pix])
     # Hessian diagonal at the ROR pixels (tomography_model.py:2872-2889).
-    hw = np.ones_like(y) if w is None else w
-    hess = np.zeros_like(x)
-    hess[system.pix] = system.A2.T @ hw
+    hess = system.hessian(np.ones_like(y) if w is None else w)
     fm = 1.0 / prm['sigma_y'] ** 2
     b = _qg.b_from_nbr_wts(prm['qggmrf_nbr_wts'])
     n_sino = y.size
@@ -238,17 +306,15 @@ def vcd_recon(system, sino, params, max_iterations, weights=None,
         sumsq = np.zeros(S)
         for k in order:
             idx = part[k]
-            cols = system.cols(idx)
-            Ak = system.A[:, cols]
             gp, hp = _qg.grad_hess_at_indices(x, rs, idx, b, prm['sigma_x'],
                                               prm['p'], prm['q'], prm['T'])
             we = e if w is None else w * e
-            fg = -fm * (Ak.T @ we)
+            fg = -fm * system.back(idx, we)
             fh = fm * hess[idx]
             d = system.direction(fg, gp, fh, hp)
             pl = np.sum(gp * d)
             pq = np.sum(hp * d * d)
-            ad = Ak @ d
+            ad = system.fwd(idx, d)
             if w is None:
                 lin, quad = fm * np.sum(e * ad), fm * np.sum(ad * ad)
             else:
@@ -257,7 +323,7 @@ def vcd_recon(system, sino, params, max_iterations, weights=None,
             alpha = min(max(alpha, F32_EPS), prm['max_alpha'])
             if prm['positivity_flag']:
                 d = np.maximum(-x[idx] / (alpha + F32_EPS), d)
-                ad = Ak @ d
+                ad = system.fwd(idx, d)
             ds = alpha * d
             # index_add_ semantics: a pixel listed twice in one subset (the
             # padding can repeat one) receives both updates.
@@ -279,5 +345,5 @@ def vcd_recon(system, sino, params, max_iterations, weights=None,
             break
     recon = x.reshape(rs[0], rs[1], S)
     if return_state:
-        return recon, stats, system.from_mat(e), hess
+        return recon, stats, system.from_vec(e), hess
     return recon, stats

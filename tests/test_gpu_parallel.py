@@ -260,3 +260,21 @@ def test_autograd_finite_differences():
         worst = max(worst, abs(fd - g) / max(1.0, abs(g)))
     print(f'finite differences: worst relative error {worst:.2e}')
     assert worst < 5e-3
+
+
+@pytest.mark.parametrize('sino_shape', [(90, 16, 64), (37, 5, 50)])
+def test_forward_ray_and_binned_gathers_agree(sino_shape):
+    """The full-list (ray) and arbitrary-list (binned) forward gathers sum the
+    same weights; they differ only in summation order."""
+    import mbirtorch_amd as mb
+    print(f'[gpu] ray vs binned {sino_shape}', flush=True)
+    m, _ = _model(sino_shape)
+    rs = m.get_params('recon_shape')
+    idx = m.full_indices_device()
+    vals = torch.rand((idx.shape[0], rs[2]), device='cuda')
+    ops = m.projector()
+    a = ops.forward(vals, idx)
+    b = ops.forward(vals, idx, runs=False)
+    err = rel_l2(a.cpu().numpy(), b.cpu().numpy())
+    print(f'ray vs binned rel L2 {err:.2e}')
+    assert err < 1e-6

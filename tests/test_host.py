@@ -144,3 +144,15 @@ def test_phantom_is_seeded_and_leaves_global_rng_alone():
     b = random_ellipsoids((32, 32, 8), 4, seed=1, axes_range=(2, 8))
     assert np.array_equal(a, b) and np.random.random() == after
     assert a.max() > 0 and a.min() >= 0
+
+
+def test_row_runs_table():
+    from mbirtorch_amd.parallel_beam import row_runs
+    import mbirtorch_amd as mb
+    rs = (12, 10, 1)
+    idx = torch.as_tensor(mb.gen_full_indices(rs))
+    runs = row_runs(idx, 12, 10)
+    assert runs is not None and int(runs[:, 2].sum()) == idx.shape[0]
+    np.random.seed(0)
+    sub = torch.as_tensor(mb.gen_pixel_partition(rs, 4)[0].astype(np.int64))
+    assert row_runs(sub, 12, 10) is None

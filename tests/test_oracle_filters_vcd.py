@@ -98,7 +98,7 @@ def test_quadratic_vcd_converges_to_normal_equations():
     sysm = vcd.ParallelSystem(angles, geom, rs)
     x_true = np.zeros((rs[0] * rs[1], R))
     x_true[sysm.pix] = rng.random((len(sysm.pix), R))
-    y = sysm.from_mat(sysm.A @ x_true[sysm.pix]).astype(np.float32)
+    y = sysm.from_vec(sysm.A @ x_true[sysm.pix]).astype(np.float32)
     sy, sx = 0.5, 0.3
     params = dict(p=2.0, q=2.0, T=1.0, sigma_y=sy, sigma_x=sx, sigma_prox=sx,
                   auto_regularize_flag=False, granularity=[1, 4], partition_sequence=[1],
@@ -133,7 +133,7 @@ def test_quadratic_vcd_converges_to_normal_equations():
     keep = np.nonzero(ror.ravel())[0]
     M = (A_full.T @ A_full) / sy ** 2 + Hp
     M = M[keep][:, keep]
-    rhs = (A_full.T @ sysm.to_mat(y).ravel()) / sy ** 2
+    rhs = (A_full.T @ sysm.to_vec(y)) / sy ** 2
     rhs = rhs[keep]
     sol, info = spla.cg(M, rhs, rtol=1e-12, maxiter=5000)
     assert info == 0
