@@ -28,6 +28,8 @@ import numpy as np
 V, R, C = 720, 64, 512
 NX = 512
 FP32_PEAK_TFLOPS = 157.3
+METRIC = ('Gvox\u00b7views/s fwd+back; recons/s + p50/p99 latency batched vs plain '
+          '@NRMSE\u22641e-3')
 HBM_PEAK_GBS = 8000.0
 
 
@@ -39,6 +41,9 @@ def parse():
     ap.add_argument('--vcd-iters', type=int, default=10)
     ap.add_argument('--no-vcd', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='also run a plain 256 MiB device copy (the reference kernel for '
+                         'scaling HBM counters) three times')
     ap.add_argument('--cpu-views', type=int, default=24,
                     help='views of the workload the CPU baseline projects')
     return ap.parse_args()
@@ -46,6 +51,20 @@ def parse():
 
 def log(msg):
     print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain
+    copy of known size), or None when it has not been measured."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                        'hbm_traffic.json')
+    try:
+        with open(path) as f:
+            entry = json.load(f)['kernels'].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if entry is None else entry['bytes_per_launch']
 
 
 def cpu_baseline(angles, vol, sino, views):
@@ -63,7 +82,7 @@ def cpu_baseline(angles, vol, sino, views):
     op.back(sino[sel], idx, angles[sel], geom)
     dt = time.perf_counter() - t0
     work = 2.0 * rs[0] * rs[1] * rs[2] * views
-    return dict(value=work / dt / 1e9, unit='Gvox*views/s', cores=1, kind='port',
+    return dict(value=work / dt / 1e9, unit='Gvox\u00b7views/s', cores=1, kind='port',
                 sample=f'oracle forward+back (FP64 sparse matrix, scipy, 1 thread) of {views} '
                        f'of the {V} views, full 512x512x64 grid: {dt:.1f} s')
 
@@ -140,8 +159,17 @@ def main():
     achieved = flops / (dom_ms * 1e-3) / 1e12
     bytes_alg = 4.0 * (P * rs[2] + V * R * C)
     roofline = dict(bound='mfma', achieved=round(achieved, 3), peak=FP32_PEAK_TFLOPS,
-                    unit='TFLOP/s', frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=None,
+                    unit='TFLOP/s', frac=round(achieved / FP32_PEAK_TFLOPS, 4),
+                    traffic=measured_traffic(f'par_{dom_name}'),
                     kernel=f'par_{dom_name}', hbm_gbs=round(bytes_alg / (dom_ms * 1e-3) / 1e9, 1))
+
+    if args.copy_calibration:
+        log('copy calibration')
+        src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device='cuda')  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
 
     vcd_ms = None
     if not args.no_vcd:
@@ -161,8 +189,8 @@ def main():
         cpu = cpu_baseline(angles, vol_np, sino_np, args.cpu_views)
 
     out = {
-        'metric': 'Gvox*views/s fwd+back (parallel 720v 64x512 det, 512x512x64 vol)',
-        'value': round(value, 2), 'unit': 'Gvox*views/s', 'n_gpus': 1,
+        'metric': METRIC,
+        'value': round(value, 2), 'unit': 'Gvox\u00b7views/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(ms_step, 4),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
         'data': 'synthetic (seeded 40-ellipsoid phantom, forward projection + 1% noise)',

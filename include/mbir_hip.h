@@ -163,22 +163,24 @@ int32_t mbir_par_fwd_partials_count(const mbir_par_geom* geom);
 /* Forward projection of the subset direction delta fused with the line
  * search reductions lin = sum(w * e * A delta), quad = sum(w * (A delta)^2)
  * (w == NULL means unit weights), FP64, one (lin, quad) pair per workgroup
- * into partials[2 * mbir_par_fwd_partials_count].  Replaces
- * sparse_forward_project(delta) + _forward_lin_quad_const/_weighted
- * (tomography_model.py:2592-2610, 65-74). */
+ * into partials[2 * mbir_par_fwd_partials_count] (pairs beyond the
+ * workgroups that ran are written as zeros).  `row_runs` as for
+ * mbir_par_forward.  Replaces sparse_forward_project(delta) +
+ * _forward_lin_quad_const/_weighted (tomography_model.py:2592-2610, 65-74). */
 int mbir_par_fwd_linesearch(const float* delta, const int64_t* pixel_indices,
-                            int32_t num_pixels, const float* view_table,
-                            const mbir_par_geom* geom, const float* error_vcr,
-                            const float* weights_vcr, double* partials,
-                            mbir_stream_t stream);
+                            int32_t num_pixels, const int32_t* row_runs,
+                            const float* view_table, const mbir_par_geom* geom,
+                            const float* error_vcr, const float* weights_vcr,
+                            double* partials, mbir_stream_t stream);
 
-/* error -= alpha[0] * (A delta), recomputing A delta tile by tile (no
- * sinogram-sized transient).  Replaces error_sinogram.sub_(alpha *
- * delta_sinogram) in _apply_update (tomography_model.py:77-87). */
+/* error -= alpha[0] * (A delta), recomputing A delta in registers (no
+ * sinogram-sized transient).  `row_runs` as for mbir_par_forward.  Replaces
+ * error_sinogram.sub_(alpha * delta_sinogram) in _apply_update
+ * (tomography_model.py:77-87). */
 int mbir_par_fwd_update(const float* delta, const int64_t* pixel_indices,
-                        int32_t num_pixels, const float* view_table,
-                        const mbir_par_geom* geom, const float* alpha,
-                        float* error_vcr, mbir_stream_t stream);
+                        int32_t num_pixels, const int32_t* row_runs,
+                        const float* view_table, const mbir_par_geom* geom,
+                        const float* alpha, float* error_vcr, mbir_stream_t stream);
 
 /* ---- cone-beam projectors ---------------------------------------------- */
 
@@ -269,13 +271,18 @@ int mbir_vcd_positivity(const float* flat_recon, const int64_t* pixel_indices,
                         int32_t num_pixels, int32_t num_slices,
                         const float* alpha, float* delta, mbir_stream_t stream);
 
+/* Number of doubles of scratch mbir_vcd_apply needs. */
+int64_t mbir_vcd_apply_scratch(int32_t num_pixels, int32_t num_slices);
+
 /* x[pix] += alpha*delta, then acc[s] += sum_p (alpha*delta[p, s])^2 for
- * each slice s and acc[num_slices] += sum |alpha*delta|, FP64, each sum owned
- * by one workgroup (deterministic).  Replaces _apply_update's recon half and
- * its per-slice / ell1 reductions (tomography_model.py:77-87). */
+ * each slice s and acc[num_slices] += sum |alpha*delta|, FP64: per-workgroup
+ * partials into `scratch`, then a fixed-order reduction (deterministic).
+ * Replaces _apply_update's recon half and its per-slice / ell1 reductions
+ * (tomography_model.py:77-87). */
 int mbir_vcd_apply(float* flat_recon, const int64_t* pixel_indices,
                    int32_t num_pixels, int32_t num_slices, const float* delta,
-                   const float* alpha, double* acc, mbir_stream_t stream);
+                   const float* alpha, double* acc, double* scratch,
+                   mbir_stream_t stream);
 
 /* ---- reductions -------------------------------------------------------- */
 

@@ -79,8 +79,8 @@ SIGNATURES = {
     'mbir_par_back': (ctypes.c_int, [_P, _P, _P, _I32, _P, _GP, _I32, _I32, _P, _P]),
     'mbir_reduce_splits': (ctypes.c_int, [_P, _I32, _I64, _P, _P]),
     'mbir_par_fwd_partials_count': (_I32, [_GP]),
-    'mbir_par_fwd_linesearch': (ctypes.c_int, [_P, _P, _I32, _P, _GP, _P, _P, _P, _P]),
-    'mbir_par_fwd_update': (ctypes.c_int, [_P, _P, _I32, _P, _GP, _P, _P, _P]),
+    'mbir_par_fwd_linesearch': (ctypes.c_int, [_P, _P, _I32, _P, _P, _GP, _P, _P, _P, _P]),
+    'mbir_par_fwd_update': (ctypes.c_int, [_P, _P, _I32, _P, _P, _GP, _P, _P, _P]),
     'mbir_cone_forward': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _I32, _P, _P]),
     'mbir_cone_back_splits': (_I32, [_CP, _I32]),
     'mbir_cone_back': (ctypes.c_int, [_P, _P, _P, _I32, _P, _CP, _I32, _I32, _P, _P]),
@@ -94,7 +94,8 @@ SIGNATURES = {
     'mbir_vcd_direction': (ctypes.c_int, [_P, _P, _I32, _P, _I32, _P, _P, _VP, _P, _P, _P]),
     'mbir_vcd_alpha': (ctypes.c_int, [_P, _I32, _P, _I32, _VP, _P, _P, _P]),
     'mbir_vcd_positivity': (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P]),
-    'mbir_vcd_apply': (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P, _P]),
+    'mbir_vcd_apply_scratch': (ctypes.c_int64, [_I32, _I32]),
+    'mbir_vcd_apply': (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P, _P, _P]),
     'mbir_reduce_blocks': (_I32, [_I64]),
     'mbir_sumsq_f64': (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
     'mbir_sumabs_f64': (ctypes.c_int, [_P, _I64, _P, _P, _P]),

@@ -86,7 +86,12 @@ class ConeProjector:
     def fwd_partials_count(self):
         return int(_lib.lib().mbir_cone_fwd_partials_count(self.geom))
 
-    def forward(self, values, idx, layout=_lib.MBIR_LAYOUT_VRC):
+    def forward(self, values, idx, layout=_lib.MBIR_LAYOUT_VRC, runs=None):
+        """Forward projection of cylinders ``values`` at ``idx``.  ``runs`` (the
+        image-row run table the parallel projector uses to pick its gather) is
+        accepted for the shared projector interface; the cone forward has one
+        form."""
+        del runs
         _lib.require_gpu(values)
         values = values.contiguous()
         if values.dim() != 2 or values.shape[1] != self.num_slices:
@@ -128,13 +133,15 @@ class ConeProjector:
                                                  _lib.stream_ptr()), 'reduce splits')
         return out
 
-    def fwd_linesearch(self, delta, idx, e_vcr, w_vcr, partials):
+    def fwd_linesearch(self, delta, idx, e_vcr, w_vcr, partials, runs=None):
+        del runs  # one cone forward form
         _lib.check(_lib.lib().mbir_cone_fwd_linesearch(
             _lib.ptr(delta), _lib.ptr(idx), int(idx.shape[0]), _lib.ptr(self.table),
             self.geom, _lib.ptr(e_vcr), _lib.ptr(w_vcr), _lib.ptr(partials),
             _lib.stream_ptr()), 'cone forward line search')
 
-    def fwd_update(self, delta, idx, alpha, e_vcr):
+    def fwd_update(self, delta, idx, alpha, e_vcr, runs=None):
+        del runs
         _lib.check(_lib.lib().mbir_cone_fwd_update(
             _lib.ptr(delta), _lib.ptr(idx), int(idx.shape[0]), _lib.ptr(self.table),
             self.geom, _lib.ptr(alpha), _lib.ptr(e_vcr), _lib.stream_ptr()),

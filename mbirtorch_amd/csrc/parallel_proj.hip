@@ -177,62 +177,116 @@ __global__ void reduce_splits_kernel(const float* __restrict__ parts, int ns,
 }
 
 // ---------------------------------------------------------------------------
-// Shared forward helpers: accumulate RPT rows of one voxel cylinder.
-template <int RPT, bool VEC>
-__device__ __forceinline__ void acc_rows(float (&acc)[RPT], float A, const float* xp, int nr) {
-  if (VEC) {
-#pragma unroll
-    for (int k = 0; k < RPT; k += 4) {
-      if (k < nr) {
-        const float4 x4 = *reinterpret_cast<const float4*>(xp + k);
-        acc[k] = fmaf(A, x4.x, acc[k]);
-        acc[k + 1] = fmaf(A, x4.y, acc[k + 1]);
-        acc[k + 2] = fmaf(A, x4.z, acc[k + 2]);
-        acc[k + 3] = fmaf(A, x4.w, acc[k + 3]);
-      }
-    }
+// Forward projectors.  Lane layout shared by both forms: a gather unit is LPC
+// consecutive lanes of a wavefront, and lane rl of a unit owns detector rows
+// rr = r0 + rl*VR .. rr+VR-1 of a row chunk of ROWS = LPC*VR rows.  A voxel
+// cylinder segment is then one contiguous LPC*VR*4-byte load per unit, and a
+// channel-major sinogram row segment (v, n, r0 : r0+ROWS) one contiguous
+// store, as in the back projector.
+template <int LPC, int VR>
+struct FwdLanes {
+  static constexpr int SPW = MBIR_WAVE / LPC;  // units per wavefront
+  static constexpr int UNITS = kBlock / LPC;   // units per workgroup
+  static constexpr int ROWS = LPC * VR;        // rows per chunk
+};
+
+template <int VR>
+__device__ __forceinline__ void load_vr(const float* p, float (&x)[VR]) {
+  if constexpr (VR == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    x[0] = t.x;
+    x[1] = t.y;
+    x[2] = t.z;
+    x[3] = t.w;
   } else {
 #pragma unroll
-    for (int k = 0; k < RPT; ++k)
-      if (k < nr) acc[k] = fmaf(A, xp[k], acc[k]);
+    for (int q = 0; q < VR; ++q) x[q] = p[q];
+  }
+}
+
+template <int VR>
+__device__ __forceinline__ void store_vr(float* p, const float (&x)[VR]) {
+  if constexpr (VR == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(x[0], x[1], x[2], x[3]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < VR; ++q) p[q] = x[q];
   }
 }
 
 enum FwdMode { FWD_WRITE_VCR = 0, FWD_WRITE_VRC = 1, FWD_LINESEARCH = 2, FWD_UPDATE = 3 };
 
-// Epilogue for a thread owning sinogram entries (v, n, r0 .. r0+nr).
-template <int RPT, int MODE, bool WEIGHTED>
-__device__ __forceinline__ void fwd_epilogue(const float (&acc)[RPT], int v, int n, int r0, int nr,
-                                             const mbir_par_geom& g, float* __restrict__ sino,
-                                             const float* __restrict__ wts, float al,
-                                             double& lin, double& quad) {
-  const int R = g.num_slices, C = g.num_channels;
-  if (MODE == FWD_WRITE_VRC) {
+// Channel-major epilogue of one lane: sinogram entries o .. o+VR-1 hold
+// (v, n, rr .. rr+VR-1).  Line search: FP64 sums of (w*e)*t and (t*t)*w;
+// update: e -= alpha*t.
+template <int VR, int MODE, bool WEIGHTED>
+__device__ __forceinline__ void fwd_epilogue_vcr(const float (&t)[VR], size_t o,
+                                                 float* __restrict__ sino,
+                                                 const float* __restrict__ wts, float al,
+                                                 double& lin, double& quad) {
+  if (MODE == FWD_WRITE_VCR) {
+    store_vr<VR>(sino + o, t);
+  } else if (MODE == FWD_LINESEARCH) {
+    float e[VR];
+    load_vr<VR>(sino + o, e);
+    if (WEIGHTED) {
+      float w[VR];
+      load_vr<VR>(wts + o, w);
 #pragma unroll
-    for (int k = 0; k < RPT; ++k)
-      if (k < nr) sino[((size_t)v * R + r0 + k) * C + n] = acc[k];
-    return;
-  }
-  const size_t o = ((size_t)v * C + n) * R + r0;
-#pragma unroll
-  for (int k = 0; k < RPT; ++k) {
-    if (k >= nr) continue;
-    const float t = acc[k];
-    if (MODE == FWD_WRITE_VCR) {
-      sino[o + k] = t;
-    } else if (MODE == FWD_LINESEARCH) {
-      const float e = sino[o + k];
-      if (WEIGHTED) {
-        const float w = wts[o + k];
-        lin += (double)((w * e) * t);
-        quad += (double)((t * t) * w);
-      } else {
-        lin += (double)(e * t);
-        quad += (double)(t * t);
+      for (int q = 0; q < VR; ++q) {
+        lin += (double)((w[q] * e[q]) * t[q]);
+        quad += (double)((t[q] * t[q]) * w[q]);
       }
     } else {
-      sino[o + k] = sino[o + k] - al * t;
+#pragma unroll
+      for (int q = 0; q < VR; ++q) {
+        lin += (double)(e[q] * t[q]);
+        quad += (double)(t[q] * t[q]);
+      }
     }
+  } else if (MODE == FWD_UPDATE) {
+    float e[VR];
+    load_vr<VR>(sino + o, e);
+#pragma unroll
+    for (int q = 0; q < VR; ++q) e[q] = e[q] - al * t[q];
+    store_vr<VR>(sino + o, e);
+  }
+}
+
+// Line-search partials: one (lin, quad) pair per workgroup at its linear
+// block index; workgroup 0 zeroes the pairs [num_blocks, n_partials) so the
+// caller always reduces n_partials pairs whichever form ran.
+__device__ __forceinline__ void fwd_write_partials(double lin, double quad,
+                                                   double* __restrict__ partials,
+                                                   int n_partials) {
+  block_sum2_f64(lin, quad);
+  const int nb = gridDim.x * gridDim.y * gridDim.z;
+  const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  if (threadIdx.x == 0) {
+    partials[2 * b] = lin;
+    partials[2 * b + 1] = quad;
+  }
+  if (b == 0) {
+    for (int i = nb + threadIdx.x; i < n_partials; i += kBlock) {
+      partials[2 * i] = 0.0;
+      partials[2 * i + 1] = 0.0;
+    }
+  }
+}
+
+// Geometry of one (view, pixel) for the forward forms: NT-wide tap window
+// (two-tap rule, or round(n_p) +- psf with `ntaps` = 2*psf+1 <= NT live taps).
+template <int NT, bool TWO>
+__device__ __forceinline__ void fwd_taps(int64_t pix, const ParView& vw, const mbir_par_geom& g,
+                                         int ntaps, int& base, float (&a)[NT]) {
+  const int row = (int)(pix / g.num_cols);
+  const int col = (int)(pix - (int64_t)row * g.num_cols);
+  const float n_p = par_np(row, col, vw, g);
+  base = par_base<TWO ? 2 : 0>(n_p, g);
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    const int n = base + k;
+    a[k] = (k < ntaps && n >= 0 && n < g.num_channels) ? par_weight(n_p, n, vw) : 0.0f;
   }
 }
 
@@ -240,69 +294,183 @@ __device__ __forceinline__ void fwd_epilogue(const float (&acc)[RPT], int v, int
 // Ray-driven forward for row-structured pixel lists.  runs[3*i .. 3*i+2] =
 // (list position of the run's first pixel, first column, run length) for
 // image row i; a length of 0 means the row holds no listed pixel.
-constexpr int kRayBlock = 128;
+//
+// Workgroup = (CHW channels, one view, one row chunk); a unit owns CPS
+// adjacent channels n0 .. n0+CPS-1 and walks the image rows (or columns,
+// whichever the view's projection is steeper in).  The candidate pixels of
+// each image row are the columns whose footprint can reach the unit's
+// channels, from the inverse map with a 0.01-pixel margin; each candidate's
+// n_p is evaluated once with the exact chain and its tap weights for the
+// CPS channels follow the same rule as the back projector, so every nonzero
+// (pixel, channel) weight is found and weights agree bit for bit.
+constexpr float kRayMargin = 0.01f;
 
-template <int RPT, int NT, bool VEC, int MODE>
-__global__ __launch_bounds__(kRayBlock) void par_fwd_ray_kernel(
-    const float* __restrict__ values, const int* __restrict__ runs,
-    const ParView* __restrict__ views, mbir_par_geom g, float* __restrict__ sino) {
-  const int n = blockIdx.x * kRayBlock + threadIdx.x;
-  const int v = blockIdx.y, r0 = blockIdx.z * RPT;
-  const int R = g.num_slices, C = g.num_channels, Nr = g.num_rows, Nc = g.num_cols;
-  if (n >= C) return;  // no barriers in this kernel
-  const int nr = min(RPT, R - r0);
-  const ParView vw = views[v];
-  float acc[RPT];
+// Four candidates (i[u], j[u]) at list positions pos[u], those with ok[u]
+// live, in one batch: geometry first, then all cylinder loads, then the FMAs,
+// so the loads of a batch are in flight together.
+template <int VR, int CPS, bool TWO>
+__device__ __forceinline__ void ray_taps4(const int (&ci)[4], const int (&cj)[4],
+                                          const int (&pos)[4], const bool (&ok)[4], int n0,
+                                          int ntaps,
+                                          bool row_ok, int rr, const ParView& vw,
+                                          const mbir_par_geom& g,
+                                          const float* __restrict__ values,
+                                          float (&acc)[CPS][VR]) {
+  float a[4][CPS];
+  bool use[4];
 #pragma unroll
-  for (int k = 0; k < RPT; ++k) acc[k] = 0.0f;
+  for (int u = 0; u < 4; ++u) {
+    const float n_p = par_np(ci[u], cj[u], vw, g);
+    const int base = par_base<TWO ? 2 : 0>(n_p, g);
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < CPS; ++k) {
+      const int t = n0 + k - base;
+      a[u][k] = (t >= 0 && t < ntaps) ? par_weight(n_p, n0 + k, vw) : 0.0f;
+      any = any || a[u][k] != 0.0f;
+    }
+    use[u] = any && row_ok && ok[u];
+  }
+  float x[4][VR];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (use[u]) {
+      load_vr<VR>(values + (size_t)pos[u] * g.num_slices + rr, x[u]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < VR; ++q) x[u][q] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int k = 0; k < CPS; ++k)
+#pragma unroll
+      for (int q = 0; q < VR; ++q) acc[k][q] = fmaf(a[u][k], x[u][q], acc[k][q]);
+}
 
-  // Candidate windows come from the inverse map in float32 with a half-pixel
-  // margin; every candidate is then checked with the exact forward chain.
-  const float x_target = ((float)n - g.det_center) * g.delta_det_channel - g.det_channel_offset;
+template <int LPC, int VR, int CPS, bool TWO, int MODE, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
+    const float* __restrict__ values, const int* __restrict__ runs,
+    const ParView* __restrict__ views, mbir_par_geom g, float* __restrict__ sino,
+    const float* __restrict__ wts, const float* __restrict__ alpha,
+    double* __restrict__ partials, int n_partials) {
+  using L = FwdLanes<LPC, VR>;
+  constexpr int CHW = L::UNITS * CPS;
+  constexpr int TS = CHW + 1;
+  __shared__ float s_tile[MODE == FWD_WRITE_VRC ? L::ROWS * TS : 1];
+
+  const int lane = threadIdx.x & (MBIR_WAVE - 1), wave = threadIdx.x / MBIR_WAVE;
+  const int unit = wave * L::SPW + lane / LPC, rl = lane % LPC;
+  const int c_base = blockIdx.x * CHW, v = blockIdx.y, r0 = blockIdx.z * L::ROWS;
+  const int R = g.num_slices, C = g.num_channels, Nr = g.num_rows, Nc = g.num_cols;
+  const int n0 = c_base + unit * CPS;
+  const int rr = r0 + rl * VR;
+  const bool row_ok = rr < R;
+  const int ntaps = TWO ? 2 : 2 * g.psf_radius + 1;
+  const ParView vw = views[v];
+
+  float acc[CPS][VR];
+#pragma unroll
+  for (int k = 0; k < CPS; ++k)
+#pragma unroll
+    for (int q = 0; q < VR; ++q) acc[k][q] = 0.0f;
+
+  const float ddc = g.delta_det_channel;
+  const float xa = ((float)n0 - g.det_center) * ddc - g.det_channel_offset;
+  const float xb = ((float)(n0 + CPS - 1) - g.det_center) * ddc - g.det_channel_offset;
   const float cd = vw.cosv * g.delta_voxel, sd = vw.sinv * g.delta_voxel_row;
   if (fabsf(cd) >= fabsf(sd)) {
-    const float hj = vw.halfw * g.delta_det_channel / fabsf(cd) + 0.5f;
+    const float hj = vw.halfw * ddc / fabsf(cd) + kRayMargin;
     for (int i = 0; i < Nr; ++i) {
       const int start = runs[3 * i], jmin = runs[3 * i + 1], cnt = runs[3 * i + 2];
       if (cnt <= 0) continue;
-      const float y_t = g.delta_voxel_row * ((float)i - g.row_center);
-      const float jstar = (x_target + vw.sinv * y_t) / cd + g.col_center;
-      const int jlo = max((int)floorf(jstar - hj), jmin);
-      const int jhi = min((int)ceilf(jstar + hj), jmin + cnt - 1);
-      for (int j = jlo; j <= jhi; ++j) {
-        float A;
-        if (par_tap_of<NT>(i, j, n, vw, g, A))
-          acc_rows<RPT, VEC>(acc, A, values + (size_t)(start + j - jmin) * R + r0, nr);
+      const float sy = vw.sinv * (g.delta_voxel_row * ((float)i - g.row_center));
+      const float ja = (xa + sy) / cd + g.col_center, jb = (xb + sy) / cd + g.col_center;
+      const int jlo = max((int)floorf(fminf(ja, jb) - hj), jmin);
+      const int jhi = min((int)ceilf(fmaxf(ja, jb) + hj), jmin + cnt - 1);
+      for (int j = jlo; j <= jhi; j += 4) {
+        int ci[4], cj[4], pos[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int jj = min(j + u, jhi);
+          ci[u] = i;
+          cj[u] = jj;
+          pos[u] = start + jj - jmin;
+          ok[u] = j + u <= jhi;
+        }
+        ray_taps4<VR, CPS, TWO>(ci, cj, pos, ok, n0, ntaps, row_ok, rr, vw, g, values, acc);
       }
     }
   } else {
-    const float hi = vw.halfw * g.delta_det_channel / fabsf(sd) + 0.5f;
+    const float hi = vw.halfw * ddc / fabsf(sd) + kRayMargin;
     for (int j = 0; j < Nc; ++j) {
-      const float x_t = g.delta_voxel * ((float)j - g.col_center);
-      const float istar = (vw.cosv * x_t - x_target) / sd + g.row_center;
-      const int ilo = max((int)floorf(istar - hi), 0);
-      const int ihi = min((int)ceilf(istar + hi), Nr - 1);
-      for (int i = ilo; i <= ihi; ++i) {
-        const int start = runs[3 * i], jmin = runs[3 * i + 1], cnt = runs[3 * i + 2];
-        if (j < jmin || j >= jmin + cnt) continue;
-        float A;
-        if (par_tap_of<NT>(i, j, n, vw, g, A))
-          acc_rows<RPT, VEC>(acc, A, values + (size_t)(start + j - jmin) * R + r0, nr);
+      const float cx = vw.cosv * (g.delta_voxel * ((float)j - g.col_center));
+      const float ia = (cx - xa) / sd + g.row_center, ib = (cx - xb) / sd + g.row_center;
+      const int ilo = max((int)floorf(fminf(ia, ib) - hi), 0);
+      const int ihi = min((int)ceilf(fmaxf(ia, ib) + hi), Nr - 1);
+      for (int i = ilo; i <= ihi; i += 4) {
+        int ci[4], cj[4], pos[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int ii = min(i + u, ihi);
+          const int start = runs[3 * ii], jmin = runs[3 * ii + 1], cnt = runs[3 * ii + 2];
+          ci[u] = ii;
+          cj[u] = j;
+          pos[u] = start + j - jmin;
+          ok[u] = i + u <= ihi && j >= jmin && j < jmin + cnt;
+        }
+        ray_taps4<VR, CPS, TWO>(ci, cj, pos, ok, n0, ntaps, row_ok, rr, vw, g, values, acc);
       }
     }
   }
-  double lin = 0.0, quad = 0.0;
-  fwd_epilogue<RPT, MODE, false>(acc, v, n, r0, nr, g, sino, nullptr, 0.0f, lin, quad);
+
+  if (MODE == FWD_WRITE_VRC) {
+    // Stage the (ROWS x CHW) tile so each detector row is stored contiguously.
+    if (row_ok) {
+#pragma unroll
+      for (int k = 0; k < CPS; ++k)
+#pragma unroll
+        for (int q = 0; q < VR; ++q) s_tile[(rl * VR + q) * TS + unit * CPS + k] = acc[k][q];
+    }
+    __syncthreads();
+    const int nrow = min(L::ROWS, R - r0), nch = min(CHW, C - c_base);
+    for (int e = threadIdx.x; e < L::ROWS * CHW; e += kBlock) {
+      const int row = e / CHW, c = e - row * CHW;
+      if (row < nrow && c < nch)
+        sino[((size_t)v * R + r0 + row) * C + c_base + c] = s_tile[row * TS + c];
+    }
+  } else {
+    double lin = 0.0, quad = 0.0;
+    const float al = MODE == FWD_UPDATE ? alpha[0] : 0.0f;
+    if (row_ok) {
+#pragma unroll
+      for (int k = 0; k < CPS; ++k) {
+        const int n = n0 + k;
+        if (n < C)
+          fwd_epilogue_vcr<VR, MODE, WEIGHTED>(acc[k], ((size_t)v * C + n) * R + rr, sino, wts,
+                                               al, lin, quad);
+      }
+    }
+    if (MODE == FWD_LINESEARCH) fwd_write_partials(lin, quad, partials, n_partials);
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Binned forward for arbitrary pixel lists.  Workgroup = (view, 256 channels,
-// RPT rows); thread t owns channel c0 + t.
-template <int NT>
+// Binned forward for arbitrary pixel lists.  Workgroup = (view, 256
+// channels, row chunk).  Each chunk of the list is counting-sorted by channel
+// in LDS (histogram, scan, scatter), every bin is then put in list order (an
+// insertion sort; bins are short), and unit u gathers the bins of channels
+// c0 + u + k*UNITS, k < LPC, accumulating in registers across chunks.  The
+// summation order of every sinogram entry is the list order: deterministic.
+template <bool TWO>
 struct BinCfg {
-  static constexpr int CH = NT <= 2 ? 1024 : (NT <= 3 ? 512 : 256);  // pixels per chunk
-  static constexpr int PPT = CH / kBlock;                             // pixels per thread
-  static constexpr int ENT = CH * NT;                                 // entry capacity
+  static constexpr int NT = TWO ? 2 : kMaxTaps;
+  static constexpr int CH = TWO ? 1024 : 256;  // pixels per chunk
+  static constexpr int PPT = CH / kBlock;      // pixels per thread
+  static constexpr int ENT = CH * NT;          // entry capacity
 };
 
 // Exclusive block scan of one int per thread (256 threads).
@@ -321,13 +489,15 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp) {
   return prefix + x - v;
 }
 
-template <int RPT, int NT, bool VEC, int MODE, bool WEIGHTED>
+template <int LPC, int VR, bool TWO, int MODE, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
     const float* __restrict__ values, const int64_t* __restrict__ pix, int n_pix,
     const ParView* __restrict__ views, mbir_par_geom g, float* __restrict__ sino,
     const float* __restrict__ wts, const float* __restrict__ alpha,
-    double* __restrict__ partials) {
-  using Cfg = BinCfg<NT>;
+    double* __restrict__ partials, int n_partials) {
+  using L = FwdLanes<LPC, VR>;
+  using Cfg = BinCfg<TWO>;
+  constexpr int NT = Cfg::NT;
   __shared__ int s_cnt[kBlock + 1];
   __shared__ int s_off[kBlock + 1];
   __shared__ int s_tmp[4];
@@ -335,14 +505,20 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
   __shared__ float s_ea[Cfg::ENT];
 
   const int tid = threadIdx.x;
-  const int v = blockIdx.x, c0 = blockIdx.y * kBlock, r0 = blockIdx.z * RPT;
+  const int lane = tid & (MBIR_WAVE - 1), wave = tid / MBIR_WAVE;
+  const int unit = wave * L::SPW + lane / LPC, rl = lane % LPC;
+  const int v = blockIdx.x, c0 = blockIdx.y * kBlock, r0 = blockIdx.z * L::ROWS;
   const int R = g.num_slices, C = g.num_channels;
-  const int n = c0 + tid;
-  const int nr = min(RPT, R - r0);
+  const int rr = r0 + rl * VR;
+  const bool row_ok = rr < R;
+  const int ntaps = TWO ? 2 : 2 * g.psf_radius + 1;
   const ParView vw = views[v];
-  float acc[RPT];
+
+  float acc[LPC][VR];
 #pragma unroll
-  for (int k = 0; k < RPT; ++k) acc[k] = 0.0f;
+  for (int k = 0; k < LPC; ++k)
+#pragma unroll
+    for (int q = 0; q < VR; ++q) acc[k][q] = 0.0f;
 
   for (int pc = 0; pc < n_pix; pc += Cfg::CH) {
     // Geometry of this thread's pixels; taps landing in [c0, c0+256) count.
@@ -356,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
       int base = 0;
       float a[NT];
       if (j < n_pix) {
-        par_taps<NT>(pix[j], vw, g, base, a);
+        fwd_taps<NT, TWO>(pix[j], vw, g, ntaps, base, a);
       } else {
 #pragma unroll
         for (int t = 0; t < NT; ++t) a[t] = 0.0f;
@@ -390,40 +566,69 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
       }
     }
     __syncthreads();
-    // Order this channel's bin by list position (bins are small for random
-    // subsets), then gather.
-    const int e0 = s_off[tid], e1 = s_off[tid + 1];
-    for (int e = e0 + 1; e < e1; ++e) {
-      const int kj = s_ej[e];
-      const float ka = s_ea[e];
-      int f = e - 1;
-      while (f >= e0 && s_ej[f] > kj) {
-        s_ej[f + 1] = s_ej[f];
-        s_ea[f + 1] = s_ea[f];
-        --f;
+    {  // thread tid puts bin tid in list order
+      const int e0 = s_off[tid], e1 = s_off[tid + 1];
+      for (int e = e0 + 1; e < e1; ++e) {
+        const int kj = s_ej[e];
+        const float ka = s_ea[e];
+        int f = e - 1;
+        while (f >= e0 && s_ej[f] > kj) {
+          s_ej[f + 1] = s_ej[f];
+          s_ea[f + 1] = s_ea[f];
+          --f;
+        }
+        s_ej[f + 1] = kj;
+        s_ea[f + 1] = ka;
       }
-      s_ej[f + 1] = kj;
-      s_ea[f + 1] = ka;
     }
-    if (n < C) {
-      for (int e = e0; e < e1; ++e)
-        acc_rows<RPT, VEC>(acc, s_ea[e], values + (size_t)(pc + s_ej[e]) * R + r0, nr);
+    __syncthreads();
+    if (row_ok) {
+      // Rounds over the unit's LPC bins: round r takes entry r of every bin,
+      // so up to LPC independent cylinder loads are in flight per lane.
+      const float* xb = values + (size_t)pc * R + rr;
+      int span[LPC];  // (bin start << 16) | bin length; ENT < 65536
+      int rounds = 0;
+#pragma unroll
+      for (int k = 0; k < LPC; ++k) {
+        const int ch = unit + k * L::UNITS;
+        const int e0 = s_off[ch], len = s_off[ch + 1] - e0;
+        span[k] = (e0 << 16) | len;
+        rounds = max(rounds, len);
+      }
+      for (int r = 0; r < rounds; ++r) {
+#pragma unroll
+        for (int k = 0; k < LPC; ++k) {
+          if (r < (span[k] & 0xffff)) {
+            const int e = (span[k] >> 16) + r;
+            const float a = s_ea[e];
+            float x[VR];
+            load_vr<VR>(xb + (size_t)s_ej[e] * R, x);
+#pragma unroll
+            for (int q = 0; q < VR; ++q) acc[k][q] = fmaf(a, x[q], acc[k][q]);
+          }
+        }
+      }
     }
     __syncthreads();
   }
 
   double lin = 0.0, quad = 0.0;
-  float al = 0.0f;
-  if (MODE == FWD_UPDATE) al = alpha[0];
-  if (n < C) fwd_epilogue<RPT, MODE, WEIGHTED>(acc, v, n, r0, nr, g, sino, wts, al, lin, quad);
-  if (MODE == FWD_LINESEARCH) {
-    block_sum2_f64(lin, quad);
-    if (tid == 0) {
-      const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-      partials[2 * b] = lin;
-      partials[2 * b + 1] = quad;
+  const float al = MODE == FWD_UPDATE ? alpha[0] : 0.0f;
+  if (row_ok) {
+#pragma unroll
+    for (int k = 0; k < LPC; ++k) {
+      const int n = c0 + unit + k * L::UNITS;
+      if (n >= C) continue;
+      if (MODE == FWD_WRITE_VRC) {
+#pragma unroll
+        for (int q = 0; q < VR; ++q) sino[((size_t)v * R + rr + q) * C + n] = acc[k][q];
+      } else {
+        fwd_epilogue_vcr<VR, MODE, WEIGHTED>(acc[k], ((size_t)v * C + n) * R + rr, sino, wts,
+                                             al, lin, quad);
+      }
     }
   }
+  if (MODE == FWD_LINESEARCH) fwd_write_partials(lin, quad, partials, n_partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -449,15 +654,6 @@ static inline int back_rw(int R) { return R >= 64 ? 64 : next_pow2(R); }
 
 static inline int back_pix_per_wg(int RW) {
   return (kBlock / MBIR_WAVE) * (MBIR_WAVE / RW) * (RW >= 4 ? 8 : 4);
-}
-
-// Rows per forward thread: 64, or the smallest power of two >= 4 covering R.
-static inline int fwd_rpt(int R) {
-  if (R > 32) return 64;
-  if (R > 16) return 32;
-  if (R > 8) return 16;
-  if (R > 4) return 8;
-  return 4;
 }
 
 template <int RW, int NT>
@@ -492,9 +688,9 @@ static int launch_back_nt(const float* sino, const float* w, const int64_t* pix,
     default: return launch_back_rw<1, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
   }
 }
-
-// Forward dispatch: ray kernel when `runs` is given (write modes), binned
-// kernel otherwise.
+// Forward dispatch: the ray form when `runs` is given, the binned form
+// otherwise.  Lane configuration: VR = 4 rows per lane (float4) when R % 4 == 0
+// and every row array is 16-byte aligned, else VR = 1.
 struct FwdArgs {
   int mode;
   const float* values;
@@ -507,79 +703,91 @@ struct FwdArgs {
   const float* w;
   const float* alpha;
   double* partials;
+  int n_partials;
   hipStream_t s;
 };
 
-template <int RPT, int NT, bool VEC>
-static int launch_fwd3(const FwdArgs& a) {
-  const mbir_par_geom& g = a.g;
-  if (a.runs) {
-    dim3 grid((g.num_channels + kRayBlock - 1) / kRayBlock, g.num_views,
-              (g.num_slices + RPT - 1) / RPT);
-    if (a.mode == FWD_WRITE_VRC)
-      par_fwd_ray_kernel<RPT, NT, VEC, FWD_WRITE_VRC><<<grid, kRayBlock, 0, a.s>>>(
-          a.values, a.runs, a.views, g, a.sino);
-    else
-      par_fwd_ray_kernel<RPT, NT, VEC, FWD_WRITE_VCR><<<grid, kRayBlock, 0, a.s>>>(
-          a.values, a.runs, a.views, g, a.sino);
-  } else {
-    dim3 grid(g.num_views, (g.num_channels + kBlock - 1) / kBlock,
-              (g.num_slices + RPT - 1) / RPT);
-    switch (a.mode) {
-      case FWD_WRITE_VCR:
-        par_fwd_bin_kernel<RPT, NT, VEC, FWD_WRITE_VCR, false><<<grid, kBlock, 0, a.s>>>(
-            a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
-        break;
-      case FWD_WRITE_VRC:
-        par_fwd_bin_kernel<RPT, NT, VEC, FWD_WRITE_VRC, false><<<grid, kBlock, 0, a.s>>>(
-            a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
-        break;
-      case FWD_LINESEARCH:
-        if (a.w)
-          par_fwd_bin_kernel<RPT, NT, VEC, FWD_LINESEARCH, true><<<grid, kBlock, 0, a.s>>>(
-              a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
-        else
-          par_fwd_bin_kernel<RPT, NT, VEC, FWD_LINESEARCH, false><<<grid, kBlock, 0, a.s>>>(
-              a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
-        break;
-      default:
-        par_fwd_bin_kernel<RPT, NT, VEC, FWD_UPDATE, false><<<grid, kBlock, 0, a.s>>>(
-            a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
+static inline int fwd_lpc(int R, bool vec) {
+  if (vec) return R <= 8 ? 2 : (R <= 16 ? 4 : (R <= 32 ? 8 : 16));
+  return R <= 16 ? 16 : 64;
+}
+static inline int ray_cps(int lpc) { return lpc >= 8 ? 4 : (lpc == 4 ? 2 : 1); }
+static inline int fwd_rows(int lpc, bool vec) { return vec ? 4 * lpc : lpc; }
+static inline int ray_chw(int lpc) { return (kBlock / lpc) * ray_cps(lpc); }
+
+// Workgroup count of either form (the line-search partials it writes).
+static long fwd_blocks(const mbir_par_geom& g, bool ray, bool vec) {
+  const int lpc = fwd_lpc(g.num_slices, vec);
+  const long z = (g.num_slices + fwd_rows(lpc, vec) - 1) / fwd_rows(lpc, vec);
+  const long cb = ray ? ray_chw(lpc) : kBlock;
+  return (long)g.num_views * ((g.num_channels + cb - 1) / cb) * z;
+}
+
+static long fwd_partials_count(const mbir_par_geom& g) {
+  long n = 0;
+  for (int ray = 0; ray < 2; ++ray)
+    for (int vec = 0; vec < 2; ++vec) {
+      const long b = fwd_blocks(g, ray != 0, vec != 0);
+      if (b > n) n = b;
     }
+  return n;
+}
+
+template <int LPC, int VR, bool TWO, int MODE, bool WEIGHTED>
+static void launch_fwd_kernel(const FwdArgs& a) {
+  const mbir_par_geom& g = a.g;
+  constexpr int ROWS = LPC * VR;
+  const int z = (g.num_slices + ROWS - 1) / ROWS;
+  if (a.runs) {
+    constexpr int CPS = LPC >= 8 ? 4 : (LPC == 4 ? 2 : 1);
+    constexpr int CHW = (kBlock / LPC) * CPS;
+    dim3 grid((g.num_channels + CHW - 1) / CHW, g.num_views, z);
+    par_fwd_ray_kernel<LPC, VR, CPS, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
+        a.values, a.runs, a.views, g, a.sino, a.w, a.alpha, a.partials, a.n_partials);
+  } else {
+    dim3 grid(g.num_views, (g.num_channels + kBlock - 1) / kBlock, z);
+    par_fwd_bin_kernel<LPC, VR, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
+        a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials, a.n_partials);
+  }
+}
+
+template <int LPC, int VR, bool TWO>
+static int launch_fwd_mode(const FwdArgs& a) {
+  switch (a.mode) {
+    case FWD_WRITE_VCR: launch_fwd_kernel<LPC, VR, TWO, FWD_WRITE_VCR, false>(a); break;
+    case FWD_WRITE_VRC: launch_fwd_kernel<LPC, VR, TWO, FWD_WRITE_VRC, false>(a); break;
+    case FWD_LINESEARCH:
+      if (a.w) launch_fwd_kernel<LPC, VR, TWO, FWD_LINESEARCH, true>(a);
+      else launch_fwd_kernel<LPC, VR, TWO, FWD_LINESEARCH, false>(a);
+      break;
+    default: launch_fwd_kernel<LPC, VR, TWO, FWD_UPDATE, false>(a);
   }
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }
 
-template <int RPT, int NT>
-static int launch_fwd2(const FwdArgs& a) {
-  // float4 rows need R % 4 == 0 (row chunks start at multiples of RPT >= 4)
-  // and a 16-byte aligned value array.
-  if (a.g.num_slices % 4 == 0 && (reinterpret_cast<uintptr_t>(a.values) & 15) == 0)
-    return launch_fwd3<RPT, NT, true>(a);
-  return launch_fwd3<RPT, NT, false>(a);
+template <bool TWO>
+static int launch_fwd_lanes(const FwdArgs& a, bool vec) {
+  const int lpc = fwd_lpc(a.g.num_slices, vec);
+  if (vec) {
+    switch (lpc) {
+      case 2: return launch_fwd_mode<2, 4, TWO>(a);
+      case 4: return launch_fwd_mode<4, 4, TWO>(a);
+      case 8: return launch_fwd_mode<8, 4, TWO>(a);
+      default: return launch_fwd_mode<16, 4, TWO>(a);
+    }
+  }
+  return lpc == 16 ? launch_fwd_mode<16, 1, TWO>(a) : launch_fwd_mode<64, 1, TWO>(a);
 }
 
-template <int NT>
-static int launch_fwd1(const FwdArgs& a) {
-  switch (fwd_rpt(a.g.num_slices)) {
-    case 64: return launch_fwd2<64, NT>(a);
-    case 32: return launch_fwd2<32, NT>(a);
-    case 16: return launch_fwd2<16, NT>(a);
-    case 8: return launch_fwd2<8, NT>(a);
-    default: return launch_fwd2<4, NT>(a);
-  }
+static inline bool aligned16(const void* p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
 }
 
 static int launch_fwd(const FwdArgs& a) {
-  if (a.g.two_tap) return launch_fwd1<2>(a);
-  switch (2 * a.g.psf_radius + 1) {
-    case 1: return launch_fwd1<1>(a);
-    case 3: return launch_fwd1<3>(a);
-    case 5: return launch_fwd1<5>(a);
-    case 7: return launch_fwd1<7>(a);
-    default: return launch_fwd1<9>(a);
-  }
+  const bool vec = a.g.num_slices % 4 == 0 && aligned16(a.values) && aligned16(a.sino) &&
+                   (!a.w || aligned16(a.w));
+  return a.g.two_tap ? launch_fwd_lanes<true>(a, vec) : launch_fwd_lanes<false>(a, vec);
 }
 
 }  // namespace mbir
@@ -588,7 +796,7 @@ using namespace mbir;
 
 extern "C" {
 
-int mbir_version(void) { return 2; }
+int mbir_version(void) { return 3; }
 
 int32_t mbir_par_back_splits(const mbir_par_geom* g, int32_t P) {
   if (!g || P <= 0) return 1;
@@ -640,42 +848,42 @@ int mbir_par_forward(const float* values, const int64_t* pix, int32_t P, const i
                      float* out, mbir_stream_t stream) {
   int st = check_geom(g);
   if (st) return st;
-  if (!values || !pix || !view_table || !out || P < 0) return MBIR_ERR_ARG;
+  if (!values || !view_table || !out || P < 0 || (!pix && !runs)) return MBIR_ERR_ARG;
   if (layout != MBIR_LAYOUT_VRC && layout != MBIR_LAYOUT_VCR) return MBIR_ERR_ARG;
   FwdArgs a{layout == MBIR_LAYOUT_VRC ? FWD_WRITE_VRC : FWD_WRITE_VCR,
             values, pix, P, runs, reinterpret_cast<const ParView*>(view_table), *g, out,
-            nullptr, nullptr, nullptr, (hipStream_t)stream};
+            nullptr, nullptr, nullptr, 0, (hipStream_t)stream};
   return launch_fwd(a);
 }
 
 int32_t mbir_par_fwd_partials_count(const mbir_par_geom* g) {
-  if (!g) return 0;
-  const int rpt = fwd_rpt(g->num_slices);
-  return g->num_views * ((g->num_channels + kBlock - 1) / kBlock) *
-         ((g->num_slices + rpt - 1) / rpt);
+  if (!g || check_geom(g)) return 0;
+  return (int32_t)fwd_partials_count(*g);
 }
 
 int mbir_par_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
-                            const float* view_table, const mbir_par_geom* g,
-                            const float* error_vcr, const float* w, double* partials,
-                            mbir_stream_t stream) {
+                            const int32_t* runs, const float* view_table,
+                            const mbir_par_geom* g, const float* error_vcr, const float* w,
+                            double* partials, mbir_stream_t stream) {
   int st = check_geom(g);
   if (st) return st;
-  if (!delta || !pix || !view_table || !error_vcr || !partials || P < 0) return MBIR_ERR_ARG;
-  FwdArgs a{FWD_LINESEARCH, delta, pix, P, nullptr,
+  if (!delta || !view_table || !error_vcr || !partials || P < 0 || (!pix && !runs))
+    return MBIR_ERR_ARG;
+  FwdArgs a{FWD_LINESEARCH, delta, pix, P, runs,
             reinterpret_cast<const ParView*>(view_table), *g, const_cast<float*>(error_vcr), w,
-            nullptr, partials, (hipStream_t)stream};
+            nullptr, partials, (int)fwd_partials_count(*g), (hipStream_t)stream};
   return launch_fwd(a);
 }
 
-int mbir_par_fwd_update(const float* delta, const int64_t* pix, int32_t P,
+int mbir_par_fwd_update(const float* delta, const int64_t* pix, int32_t P, const int32_t* runs,
                         const float* view_table, const mbir_par_geom* g, const float* alpha,
                         float* error_vcr, mbir_stream_t stream) {
   int st = check_geom(g);
   if (st) return st;
-  if (!delta || !pix || !view_table || !error_vcr || !alpha || P < 0) return MBIR_ERR_ARG;
-  FwdArgs a{FWD_UPDATE, delta, pix, P, nullptr, reinterpret_cast<const ParView*>(view_table),
-            *g, error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream};
+  if (!delta || !view_table || !error_vcr || !alpha || P < 0 || (!pix && !runs))
+    return MBIR_ERR_ARG;
+  FwdArgs a{FWD_UPDATE, delta, pix, P, runs, reinterpret_cast<const ParView*>(view_table), *g,
+            error_vcr, nullptr, alpha, nullptr, 0, (hipStream_t)stream};
   return launch_fwd(a);
 }
 

@@ -181,41 +181,62 @@ __global__ void vcd_positivity_kernel(const float* __restrict__ x, const int64_t
   }
 }
 
-__global__ void vcd_apply_kernel(float* __restrict__ x, const int64_t* __restrict__ pix, int P,
-                                 int S, const float* __restrict__ delta,
-                                 const float* __restrict__ alpha) {
+// x[pix[p]*S + s] += alpha*delta[p, s], fused with the per-workgroup stats
+// partials part[b*(S+1) + s] = sum_p (alpha*delta[p, s])^2 and
+// part[b*(S+1) + S] = sum |alpha*delta| over the workgroup's pixel range.
+// Lanes map to slices (coalesced cylinders), waves to pixels.
+__global__ __launch_bounds__(kThreads) void vcd_apply_kernel(
+    float* __restrict__ x, const int64_t* __restrict__ pix, int P, int S,
+    const float* __restrict__ delta, const float* __restrict__ alpha, int ppb,
+    double* __restrict__ part) {
+  __shared__ double s_red[kThreads / 64][64];
   const float a = alpha[0];
-  const int64_t total = (int64_t)P * S;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int p = (int)(i / S), s = (int)(i - (int64_t)p * S);
-    // A subset may list a pixel twice (partition padding); both updates land,
-    // as with index_add_.  The duplicates carry identical deltas, so the two
-    // atomic adds commute exactly and the result stays deterministic.
-    atomicAdd(&x[pix[p] * S + s], a * delta[i]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NW = kThreads / 64;
+  const int p0 = blockIdx.x * ppb, p1 = min(p0 + ppb, P);
+  double ell1 = 0.0;
+  for (int sc = 0; sc < S; sc += 64) {
+    const int s = sc + lane;
+    double sq = 0.0;
+    if (s < S) {
+      for (int p = p0 + wave; p < p1; p += NW) {
+        const float ds = a * delta[(size_t)p * S + s];
+        // A subset may list a pixel twice (partition padding); both updates
+        // land, as with index_add_.  The duplicates carry identical deltas,
+        // so the two atomic adds commute exactly: deterministic.
+        atomicAdd(&x[pix[p] * S + s], ds);
+        sq += (double)(ds * ds);
+        ell1 += (double)fabsf(ds);
+      }
+    }
+    s_red[wave][lane] = sq;
+    __syncthreads();
+    if (wave == 0 && s < S) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) t += s_red[w][lane];
+      part[(size_t)blockIdx.x * (S + 1) + s] = t;
+    }
+    __syncthreads();
   }
+  double unused = 0.0;
+  block_sum2_f64(ell1, unused);
+  if (threadIdx.x == 0) part[(size_t)blockIdx.x * (S + 1) + S] = ell1;
 }
 
-// Block s < S owns acc[s] = sum_p (alpha*delta[p,s])^2; block S owns the
-// ell1 sum acc[S] = sum |alpha*delta|.
-__global__ __launch_bounds__(kThreads) void vcd_stats_kernel(const float* __restrict__ delta,
-                                                             int P, int S,
-                                                             const float* __restrict__ alpha,
-                                                             double* __restrict__ acc) {
-  const float a = alpha[0];
-  const int s = blockIdx.x;
+// acc[c] += sum_b part[b*(S+1) + c], one workgroup per column c, fixed order.
+__global__ __launch_bounds__(kThreads) void vcd_stats_final_kernel(
+    const double* __restrict__ part, int nb, int S, double* __restrict__ acc) {
+  const int c = blockIdx.x;
   double v = 0.0, unused = 0.0;
-  if (s < S) {
-    for (int p = threadIdx.x; p < P; p += kThreads) {
-      const float ds = a * delta[(size_t)p * S + s];
-      v += (double)(ds * ds);
-    }
-  } else {
-    const int64_t total = (int64_t)P * S;
-    for (int64_t i = threadIdx.x; i < total; i += kThreads) v += (double)fabsf(a * delta[i]);
-  }
+  for (int b = threadIdx.x; b < nb; b += kThreads) v += part[(size_t)b * (S + 1) + c];
   block_sum2_f64(v, unused);
-  if (threadIdx.x == 0) acc[s] += v;
+  if (threadIdx.x == 0) acc[c] += v;
+}
+
+static inline int apply_blocks(int P) {
+  int nb = (P + 7) / 8;
+  return nb > 1024 ? 1024 : (nb < 1 ? 1 : nb);
 }
 
 // ---- reductions ------------------------------------------------------------
@@ -369,16 +390,21 @@ int mbir_vcd_positivity(const float* x, const int64_t* pix, int32_t P, int32_t S
   return MBIR_OK;
 }
 
+int64_t mbir_vcd_apply_scratch(int32_t P, int32_t S) {
+  if (P <= 0 || S <= 0) return 0;
+  return (int64_t)apply_blocks(P) * (S + 1);
+}
+
 int mbir_vcd_apply(float* x, const int64_t* pix, int32_t P, int32_t S, const float* delta,
-                   const float* alpha, double* acc, mbir_stream_t stream) {
-  if (!x || !pix || !delta || !alpha || !acc) return MBIR_ERR_ARG;
+                   const float* alpha, double* acc, double* scratch, mbir_stream_t stream) {
+  if (!x || !pix || !delta || !alpha || !acc || !scratch) return MBIR_ERR_ARG;
   if (P <= 0 || S <= 0) return MBIR_ERR_SHAPE;
   hipStream_t s = (hipStream_t)stream;
-  int nb = (int)(((int64_t)P * S + kThreads - 1) / kThreads);
-  if (nb > 8192) nb = 8192;
-  vcd_apply_kernel<<<nb, kThreads, 0, s>>>(x, pix, P, S, delta, alpha);
+  const int nb = apply_blocks(P);
+  const int ppb = (P + nb - 1) / nb;
+  vcd_apply_kernel<<<nb, kThreads, 0, s>>>(x, pix, P, S, delta, alpha, ppb, scratch);
   MBIR_CHECK_LAUNCH();
-  vcd_stats_kernel<<<S + 1, kThreads, 0, s>>>(delta, P, S, alpha, acc);
+  vcd_stats_final_kernel<<<S + 1, kThreads, 0, s>>>(scratch, nb, S, acc);
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }

@@ -57,6 +57,8 @@ def row_runs(idx, num_rows, num_cols):
 class ParallelProjector:
     """Device projector for fixed parallel-beam parameters."""
 
+    uses_row_runs = True
+
     def __init__(self, model, device):
         (ddc, dco, dv, vra) = model.get_params(['delta_det_channel', 'det_channel_offset',
                                                 'delta_voxel', 'voxel_row_aspect'])
@@ -144,17 +146,19 @@ class ParallelProjector:
                                                  _lib.stream_ptr()), 'reduce splits')
         return out
 
-    def fwd_linesearch(self, delta, idx, e_vcr, w_vcr, partials):
+    def fwd_linesearch(self, delta, idx, e_vcr, w_vcr, partials, runs=None):
+        """FP64 line-search partials of A(delta) against the error sinogram;
+        ``runs`` (row_runs of ``idx``, or None) selects the ray-driven form."""
         _lib.check(_lib.lib().mbir_par_fwd_linesearch(
-            _lib.ptr(delta), _lib.ptr(idx), int(idx.shape[0]), _lib.ptr(self.table),
-            self.geom, _lib.ptr(e_vcr), _lib.ptr(w_vcr), _lib.ptr(partials),
-            _lib.stream_ptr()), 'parallel forward line search')
+            _lib.ptr(delta), _lib.ptr(idx), int(idx.shape[0]), _lib.ptr(runs),
+            _lib.ptr(self.table), self.geom, _lib.ptr(e_vcr), _lib.ptr(w_vcr),
+            _lib.ptr(partials), _lib.stream_ptr()), 'parallel forward line search')
 
-    def fwd_update(self, delta, idx, alpha, e_vcr):
+    def fwd_update(self, delta, idx, alpha, e_vcr, runs=None):
         _lib.check(_lib.lib().mbir_par_fwd_update(
-            _lib.ptr(delta), _lib.ptr(idx), int(idx.shape[0]), _lib.ptr(self.table),
-            self.geom, _lib.ptr(alpha), _lib.ptr(e_vcr), _lib.stream_ptr()),
-            'parallel forward update')
+            _lib.ptr(delta), _lib.ptr(idx), int(idx.shape[0]), _lib.ptr(runs),
+            _lib.ptr(self.table), self.geom, _lib.ptr(alpha), _lib.ptr(e_vcr),
+            _lib.stream_ptr()), 'parallel forward update')
 
 
 class ParallelBeamModel(TomographyModel):

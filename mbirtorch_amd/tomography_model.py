@@ -699,6 +699,19 @@ class _VcdEngine:
         self.n_fwd = ops.fwd_partials_count()
         self.fwd_parts = torch.empty(2 * self.n_fwd, dtype=torch.float64, device=dev)
         self.alpha = torch.zeros(1, dtype=torch.float32, device=dev)
+        n_apply = int(_lib.lib().mbir_vcd_apply_scratch(max_p, self.S))
+        self.apply_scratch = torch.empty(max(n_apply, 1), dtype=torch.float64, device=dev)
+        # Image-row run tables of single-subset partitions (the whole region
+        # of reconstruction in row order) select the ray-driven forward.
+        self.runs = {}
+        if getattr(ops, 'uses_row_runs', False):
+            from .parallel_beam import row_runs
+            rs = model.get_params('recon_shape')
+            for pi, part in enumerate(partitions):
+                if int(part.shape[0]) == 1:
+                    r = row_runs(part[0], rs[0], rs[1])
+                    if r is not None:
+                        self.runs[pi] = r
         # [sumsq per slice (S), ell1, alpha_sum, wsq, sq, wsum, recon_l1]
         self.stats = torch.zeros(self.S + 6, dtype=torch.float64, device=dev)
         self.reducer = _Reducer(dev)
@@ -731,7 +744,8 @@ class _VcdEngine:
                 _lib.ptr(self.prox), self.prm, _lib.ptr(self.profile),
                 _lib.ptr(self.dc_scratch), _lib.ptr(self.delta), _lib.ptr(self.dir_parts), st),
                 'vcd dc direction')
-        self.ops.fwd_linesearch(self.delta, idx, self.e, self.w, self.fwd_parts)
+        runs = self.runs.get(pidx)
+        self.ops.fwd_linesearch(self.delta, idx, self.e, self.w, self.fwd_parts, runs=runs)
         _lib.check(lib.mbir_vcd_alpha(_lib.ptr(self.dir_parts), n_dir, _lib.ptr(self.fwd_parts),
                                       self.n_fwd, self.prm, _lib.ptr(self.alpha),
                                       _lib.ptr(self.stats[self.S + 1:]), st), 'vcd alpha')
@@ -739,10 +753,11 @@ class _VcdEngine:
             _lib.check(lib.mbir_vcd_positivity(_lib.ptr(self.x), _lib.ptr(idx), P, self.S,
                                                _lib.ptr(self.alpha), _lib.ptr(self.delta), st),
                        'vcd positivity')
-        self.ops.fwd_update(self.delta, idx, self.alpha, self.e)
+        self.ops.fwd_update(self.delta, idx, self.alpha, self.e, runs=runs)
         _lib.check(lib.mbir_vcd_apply(_lib.ptr(self.x), _lib.ptr(idx), P, self.S,
                                       _lib.ptr(self.delta), _lib.ptr(self.alpha),
-                                      _lib.ptr(self.stats), st), 'vcd apply')
+                                      _lib.ptr(self.stats), _lib.ptr(self.apply_scratch), st),
+                   'vcd apply')
 
     def end_iteration(self, total_sino):
         S = self.S
