@@ -127,7 +127,9 @@ int mbir_version(void);
  * the row's first listed pixel, its column, the run length; length 0 for
  * rows without listed pixels) declares that every image row's pixels form
  * one run of consecutive columns in list order; the ray-driven gather is
- * then used, otherwise the binned gather.  Replaces the forward body
+ * then used, otherwise the binned gather.  When num_slices % 4 == 0 the
+ * row arrays (values, sinogram, weights) must be 16-byte aligned
+ * (MBIR_ERR_ARG otherwise).  Replaces the forward body
  * _parallel_forward_view_batch (parallel_beam.py:56-73; Triton body
  * triton_parallel.py:598-745) driven over the whole view range by
  * Projectors.sparse_forward_project_view_range (projectors.py:518-588). */
@@ -157,15 +159,15 @@ int mbir_par_back(const float* sino_vcr, const float* weights_vcr,
 int mbir_reduce_splits(const float* parts, int32_t num_splits, int64_t n,
                        float* out, mbir_stream_t stream);
 
-/* Number of FP64 partial pairs the fused forward+line-search writes. */
-int32_t mbir_par_fwd_partials_count(const mbir_par_geom* geom);
+/* Number of FP64 partial pairs mbir_par_fwd_linesearch writes: with
+ * row_runs given (ray != 0) or without (ray == 0). */
+int32_t mbir_par_fwd_partials_count(const mbir_par_geom* geom, int32_t ray);
 
 /* Forward projection of the subset direction delta fused with the line
  * search reductions lin = sum(w * e * A delta), quad = sum(w * (A delta)^2)
  * (w == NULL means unit weights), FP64, one (lin, quad) pair per workgroup
- * into partials[2 * mbir_par_fwd_partials_count] (pairs beyond the
- * workgroups that ran are written as zeros).  `row_runs` as for
- * mbir_par_forward.  Replaces sparse_forward_project(delta) +
+ * into partials[2 * mbir_par_fwd_partials_count(geom, row_runs != NULL)].
+ * `row_runs` as for mbir_par_forward.  Replaces sparse_forward_project(delta) +
  * _forward_lin_quad_const/_weighted (tomography_model.py:2592-2610, 65-74). */
 int mbir_par_fwd_linesearch(const float* delta, const int64_t* pixel_indices,
                             int32_t num_pixels, const int32_t* row_runs,
@@ -283,6 +285,95 @@ int mbir_vcd_apply(float* flat_recon, const int64_t* pixel_indices,
                    int32_t num_pixels, int32_t num_slices, const float* delta,
                    const float* alpha, double* acc, double* scratch,
                    mbir_stream_t stream);
+
+/* ---- native VCD schedule runner ---------------------------------------- */
+
+#define MBIR_GEOMETRY_PARALLEL 0
+#define MBIR_GEOMETRY_CONE 1
+
+/* Everything one VCD reconstruction needs on the device, plus host tables of
+ * its partition subsets (subset id -> device index array, size, optional
+ * image-row run table).  Scratch sizes: back_parts >= back_parts_capacity
+ * floats (max over subsets of splits * size * slices), delta >= max size *
+ * slices floats, dir_parts >= 2 * mbir_vcd_direction_partials_count,
+ * fwd_parts >= 2 * max forward partials count, apply_scratch >=
+ * mbir_vcd_apply_scratch, red_scratch >= 3 * mbir_reduce_blocks(max(
+ * sino_size, recon_size)), dc_scratch >= mbir_vcd_direction_dc_scratch when
+ * dc_profile is set. */
+typedef struct {
+  int32_t geometry;                 /* MBIR_GEOMETRY_* */
+  int32_t positivity;               /* clip x + alpha*delta at zero */
+  int32_t num_subsets;              /* entries of the subset tables */
+  int32_t pad0;
+  const mbir_par_geom* par_geom;    /* host pointer (parallel) */
+  const mbir_cone_geom* cone_geom;  /* host pointer (cone) */
+  const float* view_table;
+  mbir_vcd_params prm;
+  float* flat_recon;                /* (rows*cols, slices) */
+  float* error_vcr;                 /* (views, channels, rows) */
+  const float* weights_vcr;         /* NULL: unit weights */
+  const float* hessian;             /* (rows*cols, slices) */
+  const float* prox;                /* proximal-map input or NULL */
+  const float* dc_profile;          /* cone DC-damping slice profile or NULL */
+  const int64_t* const* subset_pixels;
+  const int32_t* subset_sizes;
+  const int32_t* const* subset_runs; /* NULL, or per subset NULL / run table */
+  float* back_parts;
+  int64_t back_parts_capacity;
+  float* delta;
+  double* dir_parts;
+  double* fwd_parts;
+  float* alpha;
+  double* apply_scratch;
+  float* dc_scratch;
+  double* red_scratch;
+  int64_t sino_size;                /* elements of error_vcr */
+  int64_t recon_size;               /* elements of flat_recon */
+} mbir_vcd_job;
+
+/* Runs the subset steps subsets[0 .. num_steps) of one VCD iteration on
+ * `stream`, accumulating into stats (num_slices + 6 doubles, zeroed by the
+ * caller: per-slice sum (alpha*delta)^2, sum |alpha*delta|, sum alpha, then
+ * with end_iteration != 0 the forward-model sums sum w*e^2, sum e^2, sum w
+ * and sum |x|).  No host synchronisation.  Replaces the subset loop of
+ * vcd_partition_iterator / _vcd_recon (tomography_model.py:2662-2697,
+ * 2851-2960). */
+int mbir_vcd_run(const mbir_vcd_job* job, const int32_t* subsets, int32_t num_steps,
+                 double* stats, int32_t end_iteration, mbir_stream_t stream);
+
+/* ---- continuous-batching request engine -------------------------------- */
+
+/* A native worker thread with `max_active` slots, each with its own HIP
+ * stream (default priority) on the calling thread's current device.  Every
+ * round it issues one subset step of every active job (mbir_vcd_run), so the
+ * steps of different reconstructions overlap on the device; submitted jobs
+ * wait in a queue until a slot frees.  No counterpart in the reference (it
+ * has no request loop); the per-job work is exactly mbir_vcd_run's. */
+typedef struct mbir_server mbir_server;
+
+mbir_server* mbir_server_create(int32_t max_active);
+
+/* Queues a job: schedule[iter_offsets[i] .. iter_offsets[i+1]) are the subset
+ * ids of iteration i; stats holds num_iters blocks of num_slices + 6 doubles
+ * (zeroed by the caller), one per iteration, filled as for mbir_vcd_run.
+ * The job's device buffers and every host table it points to (subset
+ * tables, schedule, iter_offsets) must stay valid until the job is returned
+ * by mbir_server_poll.  The device work the job depends on must be complete
+ * (or ordered before it) when it is submitted. */
+int mbir_server_submit(mbir_server* server, const mbir_vcd_job* job,
+                       const int32_t* schedule, const int32_t* iter_offsets,
+                       int32_t num_iters, double* stats, int64_t tag);
+
+/* Up to max_jobs finished jobs (tag, status), waiting up to timeout_ms for
+ * the first; returns how many were written. */
+int32_t mbir_server_poll(mbir_server* server, int64_t* tags, int32_t* status,
+                         int32_t max_jobs, int32_t timeout_ms);
+
+/* Jobs queued or running. */
+int32_t mbir_server_in_flight(mbir_server* server);
+
+/* Waits for every job, then stops the worker and releases the streams. */
+void mbir_server_destroy(mbir_server* server);
 
 /* ---- reductions -------------------------------------------------------- */
 

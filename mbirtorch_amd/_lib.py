@@ -64,6 +64,30 @@ class VcdParams(ctypes.Structure):
                 ('num_slices', ctypes.c_int32), ('use_prox', ctypes.c_int32)]
 
 
+MBIR_GEOMETRY_PARALLEL = 0
+MBIR_GEOMETRY_CONE = 1
+
+
+class VcdJob(ctypes.Structure):
+    """mbir_vcd_job: one reconstruction's device state and subset tables."""
+    _fields_ = [('geometry', ctypes.c_int32), ('positivity', ctypes.c_int32),
+                ('num_subsets', ctypes.c_int32), ('pad0', ctypes.c_int32),
+                ('par_geom', ctypes.POINTER(ParGeom)), ('cone_geom', ctypes.POINTER(ConeGeom)),
+                ('view_table', ctypes.c_void_p), ('prm', VcdParams),
+                ('flat_recon', ctypes.c_void_p), ('error_vcr', ctypes.c_void_p),
+                ('weights_vcr', ctypes.c_void_p), ('hessian', ctypes.c_void_p),
+                ('prox', ctypes.c_void_p), ('dc_profile', ctypes.c_void_p),
+                ('subset_pixels', ctypes.POINTER(ctypes.c_void_p)),
+                ('subset_sizes', ctypes.POINTER(ctypes.c_int32)),
+                ('subset_runs', ctypes.POINTER(ctypes.c_void_p)),
+                ('back_parts', ctypes.c_void_p), ('back_parts_capacity', ctypes.c_int64),
+                ('delta', ctypes.c_void_p), ('dir_parts', ctypes.c_void_p),
+                ('fwd_parts', ctypes.c_void_p), ('alpha', ctypes.c_void_p),
+                ('apply_scratch', ctypes.c_void_p), ('dc_scratch', ctypes.c_void_p),
+                ('red_scratch', ctypes.c_void_p), ('sino_size', ctypes.c_int64),
+                ('recon_size', ctypes.c_int64)]
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -78,7 +102,7 @@ SIGNATURES = {
     'mbir_par_back_splits': (_I32, [_GP, _I32]),
     'mbir_par_back': (ctypes.c_int, [_P, _P, _P, _I32, _P, _GP, _I32, _I32, _P, _P]),
     'mbir_reduce_splits': (ctypes.c_int, [_P, _I32, _I64, _P, _P]),
-    'mbir_par_fwd_partials_count': (_I32, [_GP]),
+    'mbir_par_fwd_partials_count': (_I32, [_GP, _I32]),
     'mbir_par_fwd_linesearch': (ctypes.c_int, [_P, _P, _I32, _P, _P, _GP, _P, _P, _P, _P]),
     'mbir_par_fwd_update': (ctypes.c_int, [_P, _P, _I32, _P, _P, _GP, _P, _P, _P]),
     'mbir_cone_forward': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _I32, _P, _P]),
@@ -95,6 +119,12 @@ SIGNATURES = {
     'mbir_vcd_alpha': (ctypes.c_int, [_P, _I32, _P, _I32, _VP, _P, _P, _P]),
     'mbir_vcd_positivity': (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P]),
     'mbir_vcd_apply_scratch': (ctypes.c_int64, [_I32, _I32]),
+    'mbir_vcd_run': (ctypes.c_int, [ctypes.POINTER(VcdJob), _P, _I32, _P, _I32, _P]),
+    'mbir_server_create': (ctypes.c_void_p, [_I32]),
+    'mbir_server_submit': (ctypes.c_int, [_P, ctypes.POINTER(VcdJob), _P, _P, _I32, _P, _I64]),
+    'mbir_server_poll': (_I32, [_P, _P, _P, _I32, _I32]),
+    'mbir_server_in_flight': (_I32, [_P]),
+    'mbir_server_destroy': (None, [_P]),
     'mbir_vcd_apply': (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P, _P, _P]),
     'mbir_reduce_blocks': (_I32, [_I64]),
     'mbir_sumsq_f64': (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),

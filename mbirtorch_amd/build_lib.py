@@ -57,7 +57,7 @@ def build(verbose=False):
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(compile_one, srcs))
     tmp = LIB + '.tmp'
-    cmd = [hipcc, f'--offload-arch={ARCH}', '-shared', '-fPIC', '-o', tmp, *objs]
+    cmd = [hipcc, f'--offload-arch={ARCH}', '-shared', '-fPIC', '-pthread', '-o', tmp, *objs]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f'link failed:\n{res.stdout}\n{res.stderr}')

@@ -44,6 +44,8 @@ def _support_radius(recon_shape, dvr, dv, use_ror_mask=True):
 class ConeProjector:
     """Device projector for fixed cone-beam parameters."""
 
+    geometry_id = _lib.MBIR_GEOMETRY_CONE
+
     def __init__(self, model, device):
         V, R, C = model.get_params('sinogram_shape')
         rs = model.get_params('recon_shape')
@@ -83,7 +85,8 @@ class ConeProjector:
     def back_splits(self, P):
         return int(_lib.lib().mbir_cone_back_splits(self.geom, int(P)))
 
-    def fwd_partials_count(self):
+    def fwd_partials_count(self, ray=False):
+        del ray  # one cone forward form
         return int(_lib.lib().mbir_cone_fwd_partials_count(self.geom))
 
     def forward(self, values, idx, layout=_lib.MBIR_LAYOUT_VRC, runs=None):

@@ -141,6 +141,45 @@ __global__ __launch_bounds__(kBlock) void par_back_kernel(
         const size_t vofs = (size_t)(vb0 + b) * C * R + r;
         const float* sv = sino + vofs;
         const float* wv = WEIGHTED ? wts + vofs : nullptr;
+        if constexpr (SLOTS == 1 && NT == 2) {
+          // One pixel slot per wave: the ACC tap bases are wave-uniform.  When
+          // they span at most ACC channels (neighbouring pixels of an image
+          // row: adjacent bases differ by at most one since |cos| dv/ddc <= W
+          // <= 1), the channel window [lo, hi+1] is loaded once and every tap
+          // read from registers -- about 8|cos|+2 loads instead of 2*ACC.
+          const int q0 = b * PIX_WG + wave * PIX_WAVE;
+          int bk[ACC];
+          int lo = s_n0[q0], hi = lo;
+#pragma unroll
+          for (int k = 0; k < ACC; ++k) {
+            bk[k] = s_n0[q0 + k];
+            lo = min(lo, bk[k]);
+            hi = max(hi, bk[k]);
+          }
+          lo = __builtin_amdgcn_readfirstlane(lo);
+          hi = __builtin_amdgcn_readfirstlane(hi);
+          if (hi - lo < ACC) {
+            const int len = hi - lo + 2;
+            float win[ACC + 2];
+#pragma unroll
+            for (int m = 0; m < ACC + 2; ++m) {
+              win[m] = 0.0f;
+              if (m < len) {
+                const int n = min(max(lo + m, 0), C - 1);
+                float e = sv[(size_t)n * R];
+                if (WEIGHTED) e = e * wv[(size_t)n * R];
+                win[m] = e;
+              }
+            }
+#pragma unroll
+            for (int k = 0; k < ACC; ++k) {
+              const int d = __builtin_amdgcn_readfirstlane(bk[k] - lo);  // 0 <= d < ACC
+              acc[k] = fmaf(s_a[q0 + k], win[d], acc[k]);
+              acc[k] = fmaf(s_a[VB * PIX_WG + q0 + k], win[d + 1], acc[k]);
+            }
+            continue;
+          }
+        }
 #pragma unroll
         for (int k = 0; k < ACC; ++k) {
           const int q = b * PIX_WG + wave * PIX_WAVE + k * SLOTS + slot;
@@ -192,7 +231,16 @@ struct FwdLanes {
 
 template <int VR>
 __device__ __forceinline__ void load_vr(const float* p, float (&x)[VR]) {
-  if constexpr (VR == 4) {
+  if constexpr (VR % 4 == 0 && VR > 4) {
+#pragma unroll
+    for (int h = 0; h < VR; h += 4) {
+      const float4 t = *reinterpret_cast<const float4*>(p + h);
+      x[h] = t.x;
+      x[h + 1] = t.y;
+      x[h + 2] = t.z;
+      x[h + 3] = t.w;
+    }
+  } else if constexpr (VR == 4) {
     const float4 t = *reinterpret_cast<const float4*>(p);
     x[0] = t.x;
     x[1] = t.y;
@@ -254,23 +302,14 @@ __device__ __forceinline__ void fwd_epilogue_vcr(const float (&t)[VR], size_t o,
 }
 
 // Line-search partials: one (lin, quad) pair per workgroup at its linear
-// block index; workgroup 0 zeroes the pairs [num_blocks, n_partials) so the
-// caller always reduces n_partials pairs whichever form ran.
+// block index.
 __device__ __forceinline__ void fwd_write_partials(double lin, double quad,
-                                                   double* __restrict__ partials,
-                                                   int n_partials) {
+                                                   double* __restrict__ partials) {
   block_sum2_f64(lin, quad);
-  const int nb = gridDim.x * gridDim.y * gridDim.z;
   const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   if (threadIdx.x == 0) {
     partials[2 * b] = lin;
     partials[2 * b + 1] = quad;
-  }
-  if (b == 0) {
-    for (int i = nb + threadIdx.x; i < n_partials; i += kBlock) {
-      partials[2 * i] = 0.0;
-      partials[2 * i + 1] = 0.0;
-    }
   }
 }
 
@@ -316,17 +355,18 @@ __device__ __forceinline__ void ray_taps4(const int (&ci)[4], const int (&cj)[4]
                                           const mbir_par_geom& g,
                                           const float* __restrict__ values,
                                           float (&acc)[CPS][VR]) {
+  // The trapezoid weight is zero outside the pixel's taps (its support
+  // |n - n_p| < (W+1)/2 lies inside round(n_p) +- psf, and inside floor(n_p),
+  // floor(n_p)+1 when W <= 1), so the weight alone selects the taps.
   float a[4][CPS];
   bool use[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const float n_p = par_np(ci[u], cj[u], vw, g);
-    const int base = par_base<TWO ? 2 : 0>(n_p, g);
     bool any = false;
 #pragma unroll
     for (int k = 0; k < CPS; ++k) {
-      const int t = n0 + k - base;
-      a[u][k] = (t >= 0 && t < ntaps) ? par_weight(n_p, n0 + k, vw) : 0.0f;
+      a[u][k] = par_weight(n_p, n0 + k, vw);
       any = any || a[u][k] != 0.0f;
     }
     use[u] = any && row_ok && ok[u];
@@ -354,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
     const float* __restrict__ values, const int* __restrict__ runs,
     const ParView* __restrict__ views, mbir_par_geom g, float* __restrict__ sino,
     const float* __restrict__ wts, const float* __restrict__ alpha,
-    double* __restrict__ partials, int n_partials) {
+    double* __restrict__ partials) {
   using L = FwdLanes<LPC, VR>;
   constexpr int CHW = L::UNITS * CPS;
   constexpr int TS = CHW + 1;
@@ -454,17 +494,42 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
                                                al, lin, quad);
       }
     }
-    if (MODE == FWD_LINESEARCH) fwd_write_partials(lin, quad, partials, n_partials);
+    if (MODE == FWD_LINESEARCH) fwd_write_partials(lin, quad, partials);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Binned forward for arbitrary pixel lists.  Workgroup = (view, 256
 // channels, row chunk).  Each chunk of the list is counting-sorted by channel
-// in LDS (histogram, scan, scatter), every bin is then put in list order (an
-// insertion sort; bins are short), and unit u gathers the bins of channels
+// in LDS (histogram, scan, then a placement whose order depends only on the
+// list: pixel slot, tap, thread), and unit u gathers the bins of channels
 // c0 + u + k*UNITS, k < LPC, accumulating in registers across chunks.  The
-// summation order of every sinogram entry is the list order: deterministic.
+// summation order of every sinogram entry is fixed by the list:
+// deterministic.
+// Rank of this lane's bin b (b < 0: no entry) among the lanes below it with
+// the same bin, the size of its group, and whether it is the group's lowest
+// lane.  One ballot per distinct bin in the wavefront.
+__device__ __forceinline__ void wave_bin_rank(int b, int& rank, int& gsize, bool& leader) {
+  const int lane = threadIdx.x & (MBIR_WAVE - 1);
+  unsigned long long todo = __ballot(b >= 0);
+  rank = 0;
+  gsize = 0;
+  leader = false;
+  while (todo) {
+    const int l = __ffsll(todo) - 1;
+    const int bl = __shfl(b, l, MBIR_WAVE);
+    const unsigned long long m = __ballot(b == bl);
+    if (b == bl) {
+      rank = __popcll(m & ((1ull << lane) - 1ull));
+      gsize = __popcll(m);
+      leader = lane == l;
+    }
+    todo &= ~m;
+  }
+}
+
+constexpr int kShortBin = 24;  // longest bin the insertion sort takes
+
 template <bool TWO>
 struct BinCfg {
   static constexpr int NT = TWO ? 2 : kMaxTaps;
@@ -494,7 +559,7 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
     const float* __restrict__ values, const int64_t* __restrict__ pix, int n_pix,
     const ParView* __restrict__ views, mbir_par_geom g, float* __restrict__ sino,
     const float* __restrict__ wts, const float* __restrict__ alpha,
-    double* __restrict__ partials, int n_partials) {
+    double* __restrict__ partials) {
   using L = FwdLanes<LPC, VR>;
   using Cfg = BinCfg<TWO>;
   constexpr int NT = Cfg::NT;
@@ -503,11 +568,14 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
   __shared__ int s_tmp[4];
   __shared__ int s_ej[Cfg::ENT];
   __shared__ float s_ea[Cfg::ENT];
+  __shared__ int s_wcnt[kBlock / MBIR_WAVE][kBlock];
 
   const int tid = threadIdx.x;
   const int lane = tid & (MBIR_WAVE - 1), wave = tid / MBIR_WAVE;
   const int unit = wave * L::SPW + lane / LPC, rl = lane % LPC;
   const int v = blockIdx.x, c0 = blockIdx.y * kBlock, r0 = blockIdx.z * L::ROWS;
+#pragma unroll
+  for (int w = 0; w < kBlock / MBIR_WAVE; ++w) s_wcnt[w][tid] = 0;
   const int R = g.num_slices, C = g.num_channels;
   const int rr = r0 + rl * VR;
   const bool row_ok = rr < R;
@@ -551,22 +619,28 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
     const int off = block_exclusive_scan(mine, s_tmp);
     s_off[tid] = off;
     if (tid == kBlock - 1) s_off[kBlock] = off + mine;
-    s_cnt[tid] = off;  // scatter cursor
-    __syncthreads();
+    s_cnt[tid] = off;  // placement cursor of bin tid
+    // Two deterministic placements.  Short bins (random subsets): scatter with
+    // LDS atomics, then an insertion sort of each bin by list position.  Long
+    // bins (an image row of a dense list projecting onto one channel), where
+    // that sort is quadratic: place round by round (pixel slot q, tap t) and
+    // within a round by thread id -- the bin cursor, plus the same-bin
+    // entries of lower waves (s_wcnt), plus the rank among same-bin lanes
+    // below in the wave.
+    if (!__syncthreads_or(mine > kShortBin)) {
 #pragma unroll
-    for (int q = 0; q < Cfg::PPT; ++q) {
+      for (int q = 0; q < Cfg::PPT; ++q) {
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int b = bin[q][t];
-        if (b >= 0) {
-          const int slot = atomicAdd(&s_cnt[b], 1);
-          s_ej[slot] = tid + q * kBlock;  // position within the chunk
-          s_ea[slot] = wgt[q][t];
+        for (int t = 0; t < NT; ++t) {
+          const int b = bin[q][t];
+          if (b >= 0) {
+            const int slot = atomicAdd(&s_cnt[b], 1);
+            s_ej[slot] = tid + q * kBlock;  // position within the chunk
+            s_ea[slot] = wgt[q][t];
+          }
         }
       }
-    }
-    __syncthreads();
-    {  // thread tid puts bin tid in list order
+      __syncthreads();
       const int e0 = s_off[tid], e1 = s_off[tid + 1];
       for (int e = e0 + 1; e < e1; ++e) {
         const int kj = s_ej[e];
@@ -579,6 +653,34 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
         }
         s_ej[f + 1] = kj;
         s_ea[f + 1] = ka;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < Cfg::PPT; ++q) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const int b = bin[q][t];
+          int rank, gsize;
+          bool leader;
+          wave_bin_rank(b, rank, gsize, leader);
+          if (leader) s_wcnt[wave][b] = gsize;
+          if (!__syncthreads_or(b >= 0)) continue;
+          if (b >= 0) {
+            int slot = s_cnt[b] + rank;
+            for (int w = 0; w < wave; ++w) slot += s_wcnt[w][b];
+            s_ej[slot] = tid + q * kBlock;
+            s_ea[slot] = wgt[q][t];
+          }
+          __syncthreads();
+          int add = 0;
+#pragma unroll
+          for (int w = 0; w < kBlock / MBIR_WAVE; ++w) {
+            add += s_wcnt[w][tid];
+            s_wcnt[w][tid] = 0;
+          }
+          s_cnt[tid] += add;
+          __syncthreads();
+        }
       }
     }
     __syncthreads();
@@ -628,7 +730,7 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
       }
     }
   }
-  if (MODE == FWD_LINESEARCH) fwd_write_partials(lin, quad, partials, n_partials);
+  if (MODE == FWD_LINESEARCH) fwd_write_partials(lin, quad, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -703,34 +805,37 @@ struct FwdArgs {
   const float* w;
   const float* alpha;
   double* partials;
-  int n_partials;
   hipStream_t s;
 };
 
-static inline int fwd_lpc(int R, bool vec) {
-  if (vec) return R <= 8 ? 2 : (R <= 16 ? 4 : (R <= 32 ? 8 : 16));
-  return R <= 16 ? 16 : 64;
+// Lane configurations (lanes per unit, rows per lane).  VR = 4 or 8 rows per
+// lane (float4 loads) exactly when R % 4 == 0; the row arrays must then be
+// 16-byte aligned.
+struct LaneCfg {
+  int lpc, vr;
+};
+static inline bool fwd_vec(const mbir_par_geom& g) { return g.num_slices % 4 == 0; }
+static inline LaneCfg bin_lanes(int R, bool vec) {
+  if (vec) return {R <= 8 ? 2 : (R <= 16 ? 4 : (R <= 32 ? 8 : 16)), 4};
+  return {R <= 16 ? 16 : 64, 1};
+}
+// The ray form shares its geometry over the lanes of a unit, so it takes
+// eight rows per lane (fewer lanes per unit) for tall cylinders.
+static inline LaneCfg ray_lanes(int R, bool vec) {
+  if (vec && R > 32 && R % 8 == 0) return {8, 8};
+  return bin_lanes(R, vec);
 }
 static inline int ray_cps(int lpc) { return lpc >= 8 ? 4 : (lpc == 4 ? 2 : 1); }
-static inline int fwd_rows(int lpc, bool vec) { return vec ? 4 * lpc : lpc; }
-static inline int ray_chw(int lpc) { return (kBlock / lpc) * ray_cps(lpc); }
 
-// Workgroup count of either form (the line-search partials it writes).
-static long fwd_blocks(const mbir_par_geom& g, bool ray, bool vec) {
-  const int lpc = fwd_lpc(g.num_slices, vec);
-  const long z = (g.num_slices + fwd_rows(lpc, vec) - 1) / fwd_rows(lpc, vec);
-  const long cb = ray ? ray_chw(lpc) : kBlock;
+// Workgroup count of the ray (ray = true) or binned form: the number of
+// line-search partial pairs it writes.
+static long fwd_blocks(const mbir_par_geom& g, bool ray) {
+  const bool vec = fwd_vec(g);
+  const LaneCfg c = ray ? ray_lanes(g.num_slices, vec) : bin_lanes(g.num_slices, vec);
+  const long rows = (long)c.lpc * c.vr;
+  const long z = (g.num_slices + rows - 1) / rows;
+  const long cb = ray ? (kBlock / c.lpc) * ray_cps(c.lpc) : kBlock;
   return (long)g.num_views * ((g.num_channels + cb - 1) / cb) * z;
-}
-
-static long fwd_partials_count(const mbir_par_geom& g) {
-  long n = 0;
-  for (int ray = 0; ray < 2; ++ray)
-    for (int vec = 0; vec < 2; ++vec) {
-      const long b = fwd_blocks(g, ray != 0, vec != 0);
-      if (b > n) n = b;
-    }
-  return n;
 }
 
 template <int LPC, int VR, bool TWO, int MODE, bool WEIGHTED>
@@ -743,11 +848,11 @@ static void launch_fwd_kernel(const FwdArgs& a) {
     constexpr int CHW = (kBlock / LPC) * CPS;
     dim3 grid((g.num_channels + CHW - 1) / CHW, g.num_views, z);
     par_fwd_ray_kernel<LPC, VR, CPS, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
-        a.values, a.runs, a.views, g, a.sino, a.w, a.alpha, a.partials, a.n_partials);
+        a.values, a.runs, a.views, g, a.sino, a.w, a.alpha, a.partials);
   } else {
     dim3 grid(g.num_views, (g.num_channels + kBlock - 1) / kBlock, z);
     par_fwd_bin_kernel<LPC, VR, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
-        a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials, a.n_partials);
+        a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
   }
 }
 
@@ -768,16 +873,17 @@ static int launch_fwd_mode(const FwdArgs& a) {
 
 template <bool TWO>
 static int launch_fwd_lanes(const FwdArgs& a, bool vec) {
-  const int lpc = fwd_lpc(a.g.num_slices, vec);
-  if (vec) {
-    switch (lpc) {
+  const LaneCfg c = a.runs ? ray_lanes(a.g.num_slices, vec) : bin_lanes(a.g.num_slices, vec);
+  if (c.vr == 8) return launch_fwd_mode<8, 8, TWO>(a);
+  if (c.vr == 4) {
+    switch (c.lpc) {
       case 2: return launch_fwd_mode<2, 4, TWO>(a);
       case 4: return launch_fwd_mode<4, 4, TWO>(a);
       case 8: return launch_fwd_mode<8, 4, TWO>(a);
       default: return launch_fwd_mode<16, 4, TWO>(a);
     }
   }
-  return lpc == 16 ? launch_fwd_mode<16, 1, TWO>(a) : launch_fwd_mode<64, 1, TWO>(a);
+  return c.lpc == 16 ? launch_fwd_mode<16, 1, TWO>(a) : launch_fwd_mode<64, 1, TWO>(a);
 }
 
 static inline bool aligned16(const void* p) {
@@ -785,8 +891,9 @@ static inline bool aligned16(const void* p) {
 }
 
 static int launch_fwd(const FwdArgs& a) {
-  const bool vec = a.g.num_slices % 4 == 0 && aligned16(a.values) && aligned16(a.sino) &&
-                   (!a.w || aligned16(a.w));
+  const bool vec = fwd_vec(a.g);
+  if (vec && !(aligned16(a.values) && aligned16(a.sino) && (!a.w || aligned16(a.w))))
+    return MBIR_ERR_ARG;
   return a.g.two_tap ? launch_fwd_lanes<true>(a, vec) : launch_fwd_lanes<false>(a, vec);
 }
 
@@ -852,13 +959,13 @@ int mbir_par_forward(const float* values, const int64_t* pix, int32_t P, const i
   if (layout != MBIR_LAYOUT_VRC && layout != MBIR_LAYOUT_VCR) return MBIR_ERR_ARG;
   FwdArgs a{layout == MBIR_LAYOUT_VRC ? FWD_WRITE_VRC : FWD_WRITE_VCR,
             values, pix, P, runs, reinterpret_cast<const ParView*>(view_table), *g, out,
-            nullptr, nullptr, nullptr, 0, (hipStream_t)stream};
+            nullptr, nullptr, nullptr, (hipStream_t)stream};
   return launch_fwd(a);
 }
 
-int32_t mbir_par_fwd_partials_count(const mbir_par_geom* g) {
+int32_t mbir_par_fwd_partials_count(const mbir_par_geom* g, int32_t ray) {
   if (!g || check_geom(g)) return 0;
-  return (int32_t)fwd_partials_count(*g);
+  return (int32_t)fwd_blocks(*g, ray != 0);
 }
 
 int mbir_par_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
@@ -871,7 +978,7 @@ int mbir_par_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
     return MBIR_ERR_ARG;
   FwdArgs a{FWD_LINESEARCH, delta, pix, P, runs,
             reinterpret_cast<const ParView*>(view_table), *g, const_cast<float*>(error_vcr), w,
-            nullptr, partials, (int)fwd_partials_count(*g), (hipStream_t)stream};
+            nullptr, partials, (hipStream_t)stream};
   return launch_fwd(a);
 }
 
@@ -883,7 +990,7 @@ int mbir_par_fwd_update(const float* delta, const int64_t* pix, int32_t P, const
   if (!delta || !view_table || !error_vcr || !alpha || P < 0 || (!pix && !runs))
     return MBIR_ERR_ARG;
   FwdArgs a{FWD_UPDATE, delta, pix, P, runs, reinterpret_cast<const ParView*>(view_table), *g,
-            error_vcr, nullptr, alpha, nullptr, 0, (hipStream_t)stream};
+            error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream};
   return launch_fwd(a);
 }
 

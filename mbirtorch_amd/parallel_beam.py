@@ -58,6 +58,7 @@ class ParallelProjector:
     """Device projector for fixed parallel-beam parameters."""
 
     uses_row_runs = True
+    geometry_id = _lib.MBIR_GEOMETRY_PARALLEL
 
     def __init__(self, model, device):
         (ddc, dco, dv, vra) = model.get_params(['delta_det_channel', 'det_channel_offset',
@@ -91,8 +92,8 @@ class ParallelProjector:
     def back_splits(self, P):
         return int(_lib.lib().mbir_par_back_splits(self.geom, int(P)))
 
-    def fwd_partials_count(self):
-        return int(_lib.lib().mbir_par_fwd_partials_count(self.geom))
+    def fwd_partials_count(self, ray=False):
+        return int(_lib.lib().mbir_par_fwd_partials_count(self.geom, int(bool(ray))))
 
     def _check_values(self, values, idx):
         if values.dim() != 2 or values.shape[1] != self.num_slices:

@@ -22,6 +22,7 @@ the reference's subset order.
 """
 
 import datetime
+import ctypes
 import math
 import os
 import warnings
@@ -454,6 +455,56 @@ class TomographyModel(ParameterHandler):
                    weights=None, init_recon=None, prox_input=None, first_iteration=0,
                    init_error_sinogram=None, fm_hessian=None, return_checkpoint=False):
         """The VCD engine on the device (tomography_model.py:2699-3013)."""
+        engine, hess = self._prepare_vcd(sinogram, partitions, weights=weights,
+                                         init_recon=init_recon, prox_input=prox_input,
+                                         init_error_sinogram=init_error_sinogram,
+                                         fm_hessian=fm_hessian)
+        recon_shape, sino_shape = self.get_params(['recon_shape', 'sinogram_shape'])
+        S = recon_shape[2]
+        verbose, sigma_y = self.get_params(['verbose', 'sigma_y'])
+        total_sino = math.prod(sino_shape)
+        max_iters = partition_sequence.size
+        fm_rmse = np.zeros(max_iters)
+        nmae = np.zeros(max_iters)
+        alpha_values = np.zeros(max_iters)
+        dnorm = np.zeros((max_iters, S))
+        num_iters = 0
+        self.logger.info('Starting VCD iterations')
+        for i in range(max_iters):
+            pidx = int(partition_sequence[i])
+            num_subsets = partitions[pidx].shape[0]
+            order = np.random.permutation(num_subsets)
+            st = engine.iteration(pidx, order, total_sino)
+            fm_rmse[i] = st['fm_loss'] / sigma_y
+            nmae[i] = st['ell1'] / st['recon_l1'] if st['recon_l1'] else float('nan')
+            alpha_values[i] = st['alpha_sum'] / num_subsets
+            dnorm[i] = np.sqrt(st['sumsq'])
+            if verbose >= 1:
+                self.logger.info('\nAfter iteration {} of a max of {}: Pct change={:.4f}, '
+                                 'Forward loss={:.4f}'.format(i + first_iteration,
+                                                              max_iters + first_iteration,
+                                                              100 * nmae[i], fm_rmse[i]))
+                self.logger.info(f'Relative step size (alpha)={alpha_values[i]:.2f}, '
+                                 f"Error sino RMSE={st['es_rmse']:.4f}")
+                self.logger.info('Number subsets = {}'.format(num_subsets))
+            num_iters += 1
+            if nmae[i] < stop_threshold_change_pct / 100:
+                self.logger.warning('Change threshold stopping condition reached')
+                break
+        recon3d = engine.x.reshape(recon_shape[0], recon_shape[1], S)
+        losses = (fm_rmse[:num_iters], np.zeros(num_iters), nmae[:num_iters],
+                  alpha_values[:num_iters], dnorm[:num_iters])
+        if return_checkpoint:
+            ckpt = {'error_sinogram': self._vcr_to_vrc(engine.e), 'fm_hessian':
+                    hess.reshape(recon_shape[0], recon_shape[1], S)}
+            return recon3d, losses, ckpt
+        return recon3d, losses
+
+    def _prepare_vcd(self, sinogram, partitions, weights=None, init_recon=None, prox_input=None,
+                     init_error_sinogram=None, fm_hessian=None):
+        """Device state of a VCD run -- initial recon, error sinogram, Hessian
+        diagonal, parameters -- as a _VcdEngine (tomography_model.py:2699-2850).
+        Returns (engine, hessian)."""
         self.verify_valid_params()
         recon_shape, sino_shape = self.get_params(['recon_shape', 'sinogram_shape'])
         if tuple(np.shape(sinogram)) != tuple(sino_shape):
@@ -510,47 +561,7 @@ class TomographyModel(ParameterHandler):
 
         prm = self._vcd_params(prox=prox_input is not None)
         engine = _VcdEngine(self, ops, prm, x_flat, e_vcr, w_vcr, hess, prox_flat, partitions)
-        verbose, sigma_y = self.get_params(['verbose', 'sigma_y'])
-        total_sino = math.prod(sino_shape)
-        max_iters = partition_sequence.size
-        fm_rmse = np.zeros(max_iters)
-        nmae = np.zeros(max_iters)
-        alpha_values = np.zeros(max_iters)
-        dnorm = np.zeros((max_iters, S))
-        num_iters = 0
-        self.logger.info('Starting VCD iterations')
-        for i in range(max_iters):
-            pidx = int(partition_sequence[i])
-            num_subsets = partitions[pidx].shape[0]
-            order = np.random.permutation(num_subsets)
-            engine.begin_iteration()
-            for k in order:
-                engine.subset_step(pidx, int(k))
-            st = engine.end_iteration(total_sino)
-            fm_rmse[i] = st['fm_loss'] / sigma_y
-            nmae[i] = st['ell1'] / st['recon_l1'] if st['recon_l1'] else float('nan')
-            alpha_values[i] = st['alpha_sum'] / num_subsets
-            dnorm[i] = np.sqrt(st['sumsq'])
-            if verbose >= 1:
-                self.logger.info('\nAfter iteration {} of a max of {}: Pct change={:.4f}, '
-                                 'Forward loss={:.4f}'.format(i + first_iteration,
-                                                              max_iters + first_iteration,
-                                                              100 * nmae[i], fm_rmse[i]))
-                self.logger.info(f'Relative step size (alpha)={alpha_values[i]:.2f}, '
-                                 f"Error sino RMSE={st['es_rmse']:.4f}")
-                self.logger.info('Number subsets = {}'.format(num_subsets))
-            num_iters += 1
-            if nmae[i] < stop_threshold_change_pct / 100:
-                self.logger.warning('Change threshold stopping condition reached')
-                break
-        recon3d = engine.x.reshape(recon_shape[0], recon_shape[1], S)
-        losses = (fm_rmse[:num_iters], np.zeros(num_iters), nmae[:num_iters],
-                  alpha_values[:num_iters], dnorm[:num_iters])
-        if return_checkpoint:
-            ckpt = {'error_sinogram': self._vcr_to_vrc(engine.e), 'fm_hessian':
-                    hess.reshape(recon_shape[0], recon_shape[1], S)}
-            return recon3d, losses, ckpt
-        return recon3d, losses
+        return engine, hess
 
     def initialize_recon(self, sinogram, weights=None, init_recon=None, max_iterations=15,
                          first_iteration=0, logfile_path='~/.mbirtorch/logs/recon.log',
@@ -679,90 +690,107 @@ def generate_direct_recon_filter(num_channels, filter_name='ramp'):
 
 
 class _VcdEngine:
-    """Device-resident VCD state and the per-subset launch sequence."""
+    """Device-resident VCD state handed to the native schedule runner
+    (mbir_vcd_run): one library call per iteration issues every subset step
+    on the current stream; the host synchronises once per iteration to read
+    the statistics."""
 
     def __init__(self, model, ops, prm, x, e, w, hess, prox, partitions):
         self.model, self.ops, self.prm = model, ops, prm
         self.x, self.e, self.w, self.hess, self.prox = x, e, w, hess, prox
         self.partitions = partitions
-        self.S = prm.num_slices
-        dev = x.device
-        self.dev = dev
-        max_p = max(int(p.shape[1]) for p in partitions)
-        self.max_p = max_p
-        self.ns_max = max(ops.back_splits(int(p.shape[1])) for p in partitions)
-        self.back_parts = torch.empty(self.ns_max * max_p * self.S, dtype=torch.float32,
-                                      device=dev)
-        self.delta = torch.empty(max_p * self.S, dtype=torch.float32, device=dev)
-        self.n_dir = int(_lib.lib().mbir_vcd_direction_partials_count(max_p, self.S))
-        self.dir_parts = torch.empty(2 * self.n_dir, dtype=torch.float64, device=dev)
-        self.n_fwd = ops.fwd_partials_count()
-        self.fwd_parts = torch.empty(2 * self.n_fwd, dtype=torch.float64, device=dev)
-        self.alpha = torch.zeros(1, dtype=torch.float32, device=dev)
-        n_apply = int(_lib.lib().mbir_vcd_apply_scratch(max_p, self.S))
-        self.apply_scratch = torch.empty(max(n_apply, 1), dtype=torch.float64, device=dev)
-        # Image-row run tables of single-subset partitions (the whole region
-        # of reconstruction in row order) select the ray-driven forward.
-        self.runs = {}
+        S = self.S = prm.num_slices
+        dev = self.dev = x.device
+        lib = _lib.lib()
+        sizes = [int(p.shape[1]) for p in partitions]
+        max_p = max(sizes)
+        # Subset tables: subset id = offset[partition] + k.
+        self.offsets = np.cumsum([0] + [int(p.shape[0]) for p in partitions])[:-1]
+        n_sub = int(sum(int(p.shape[0]) for p in partitions))
+        self._pix = (ctypes.c_void_p * n_sub)()
+        self._sizes = (ctypes.c_int32 * n_sub)()
+        self._runs = (ctypes.c_void_p * n_sub)()
+        self._keep = []
+        row_runs = None
         if getattr(ops, 'uses_row_runs', False):
             from .parallel_beam import row_runs
-            rs = model.get_params('recon_shape')
-            for pi, part in enumerate(partitions):
-                if int(part.shape[0]) == 1:
-                    r = row_runs(part[0], rs[0], rs[1])
-                    if r is not None:
-                        self.runs[pi] = r
-        # [sumsq per slice (S), ell1, alpha_sum, wsq, sq, wsum, recon_l1]
-        self.stats = torch.zeros(self.S + 6, dtype=torch.float64, device=dev)
-        self.reducer = _Reducer(dev)
-        self.positivity = bool(model.get_params('positivity_flag'))
+        rs = model.get_params('recon_shape')
+        for pi, part in enumerate(partitions):
+            part = part.contiguous()
+            self._keep.append(part)
+            n_k, P = int(part.shape[0]), int(part.shape[1])
+            runs = None
+            if row_runs is not None and n_k == 1:
+                # A single-subset partition is the whole region of
+                # reconstruction in row order: the ray-driven forward applies.
+                runs = row_runs(part[0], rs[0], rs[1])
+                if runs is not None:
+                    self._keep.append(runs)
+            for k in range(n_k):
+                sid = int(self.offsets[pi]) + k
+                self._pix[sid] = part.data_ptr() + k * P * part.element_size()
+                self._sizes[sid] = P
+                self._runs[sid] = None if runs is None else runs.data_ptr()
+        back_cap = max(ops.back_splits(P) * P * S for P in sizes)
+        self.back_parts = torch.empty(back_cap, dtype=torch.float32, device=dev)
+        self.delta = torch.empty(max_p * S, dtype=torch.float32, device=dev)
+        n_dir = int(lib.mbir_vcd_direction_partials_count(max_p, S))
+        self.dir_parts = torch.empty(2 * n_dir, dtype=torch.float64, device=dev)
+        n_fwd = max(ops.fwd_partials_count(ray=False), ops.fwd_partials_count(ray=True))
+        self.fwd_parts = torch.empty(2 * n_fwd, dtype=torch.float64, device=dev)
+        self.alpha = torch.zeros(1, dtype=torch.float32, device=dev)
+        n_apply = int(lib.mbir_vcd_apply_scratch(max_p, S))
+        self.apply_scratch = torch.empty(max(n_apply, 1), dtype=torch.float64, device=dev)
+        n_red = 3 * int(lib.mbir_reduce_blocks(max(e.numel(), x.numel())))
+        self.red_scratch = torch.empty(n_red, dtype=torch.float64, device=dev)
         self.profile = model._direction_profile()
         self.dc_scratch = None
         if self.profile is not None:
-            n = int(_lib.lib().mbir_vcd_direction_dc_scratch(max_p, self.S))
+            n = int(lib.mbir_vcd_direction_dc_scratch(max_p, S))
             self.dc_scratch = torch.empty(n, dtype=torch.float32, device=dev)
+        # [sumsq per slice (S), ell1, alpha_sum, wsq, sq, wsum, recon_l1]
+        self.stats = torch.zeros(S + 6, dtype=torch.float64, device=dev)
 
-    def begin_iteration(self):
-        self.stats.zero_()
-
-    def subset_step(self, pidx, k):
-        lib, st = _lib.lib(), _lib.stream_ptr()
-        idx = self.partitions[pidx][k]
-        P = int(idx.shape[0])
-        ns = self.ops.back_splits(P)
-        parts = self.back_parts[:ns * P * self.S]
-        self.ops.back_into(self.e, idx, parts, ns, weights=self.w)
-        n_dir = int(lib.mbir_vcd_direction_partials_count(P, self.S))
-        if self.profile is None:
-            _lib.check(lib.mbir_vcd_direction(
-                _lib.ptr(self.x), _lib.ptr(idx), P, _lib.ptr(parts), ns, _lib.ptr(self.hess),
-                _lib.ptr(self.prox), self.prm, _lib.ptr(self.delta), _lib.ptr(self.dir_parts),
-                st), 'vcd direction')
+        job = _lib.VcdJob()
+        job.geometry = ops.geometry_id
+        job.positivity = int(bool(model.get_params('positivity_flag')))
+        job.num_subsets = n_sub
+        if ops.geometry_id == _lib.MBIR_GEOMETRY_PARALLEL:
+            job.par_geom = ctypes.pointer(ops.geom)
         else:
-            _lib.check(lib.mbir_vcd_direction_dc(
-                _lib.ptr(self.x), _lib.ptr(idx), P, _lib.ptr(parts), ns, _lib.ptr(self.hess),
-                _lib.ptr(self.prox), self.prm, _lib.ptr(self.profile),
-                _lib.ptr(self.dc_scratch), _lib.ptr(self.delta), _lib.ptr(self.dir_parts), st),
-                'vcd dc direction')
-        runs = self.runs.get(pidx)
-        self.ops.fwd_linesearch(self.delta, idx, self.e, self.w, self.fwd_parts, runs=runs)
-        _lib.check(lib.mbir_vcd_alpha(_lib.ptr(self.dir_parts), n_dir, _lib.ptr(self.fwd_parts),
-                                      self.n_fwd, self.prm, _lib.ptr(self.alpha),
-                                      _lib.ptr(self.stats[self.S + 1:]), st), 'vcd alpha')
-        if self.positivity:
-            _lib.check(lib.mbir_vcd_positivity(_lib.ptr(self.x), _lib.ptr(idx), P, self.S,
-                                               _lib.ptr(self.alpha), _lib.ptr(self.delta), st),
-                       'vcd positivity')
-        self.ops.fwd_update(self.delta, idx, self.alpha, self.e, runs=runs)
-        _lib.check(lib.mbir_vcd_apply(_lib.ptr(self.x), _lib.ptr(idx), P, self.S,
-                                      _lib.ptr(self.delta), _lib.ptr(self.alpha),
-                                      _lib.ptr(self.stats), _lib.ptr(self.apply_scratch), st),
-                   'vcd apply')
+            job.cone_geom = ctypes.pointer(ops.geom)
+        job.view_table = ops.table.data_ptr()
+        job.prm = prm
+        job.flat_recon = x.data_ptr()
+        job.error_vcr = e.data_ptr()
+        job.weights_vcr = None if w is None else w.data_ptr()
+        job.hessian = hess.data_ptr()
+        job.prox = None if prox is None else prox.data_ptr()
+        job.dc_profile = None if self.profile is None else self.profile.data_ptr()
+        job.subset_pixels = self._pix
+        job.subset_sizes = self._sizes
+        job.subset_runs = self._runs
+        job.back_parts = self.back_parts.data_ptr()
+        job.back_parts_capacity = back_cap
+        job.delta = self.delta.data_ptr()
+        job.dir_parts = self.dir_parts.data_ptr()
+        job.fwd_parts = self.fwd_parts.data_ptr()
+        job.alpha = self.alpha.data_ptr()
+        job.apply_scratch = self.apply_scratch.data_ptr()
+        job.dc_scratch = None if self.dc_scratch is None else self.dc_scratch.data_ptr()
+        job.red_scratch = self.red_scratch.data_ptr()
+        job.sino_size = e.numel()
+        job.recon_size = x.numel()
+        self.job = job
 
-    def end_iteration(self, total_sino):
+    def iteration(self, pidx, order, total_sino):
+        """All subset steps of one iteration in `order`, then the statistics."""
         S = self.S
-        self.reducer.sumsq(self.e, self.w, self.stats[S + 2:S + 5])
-        self.reducer.sumabs(self.x, self.stats[S + 5:S + 6])
+        ids = (np.asarray(order, dtype=np.int64) + int(self.offsets[pidx])).astype(np.int32)
+        self.stats.zero_()
+        _lib.check(_lib.lib().mbir_vcd_run(
+            ctypes.byref(self.job), ids.ctypes.data_as(ctypes.c_void_p), int(ids.size),
+            _lib.ptr(self.stats), 1, _lib.stream_ptr()), 'vcd iteration')
         h = self.stats.cpu().numpy()
         wsq, sq, wsum = h[S + 2], h[S + 3], h[S + 4]
         avg_w = 1.0 if self.w is None else wsum / total_sino

@@ -46,7 +46,8 @@ def test_size_queries_need_no_gpu():
                      delta_det_channel=1.0, delta_voxel=1.0, delta_voxel_row=1.0)
     assert lib.mbir_par_back_splits(g, 205012) == 1
     assert lib.mbir_par_back_splits(g, 1602) > 1
-    assert lib.mbir_par_fwd_partials_count(g) > 0
+    assert lib.mbir_par_fwd_partials_count(g, 0) == 720 * 2
+    assert lib.mbir_par_fwd_partials_count(g, 1) == 720 * 4
     assert lib.mbir_reduce_blocks(10 ** 6) >= 1
 
 
@@ -156,3 +157,52 @@ def test_row_runs_table():
     np.random.seed(0)
     sub = torch.as_tensor(mb.gen_pixel_partition(rs, 4)[0].astype(np.int64))
     assert row_runs(sub, 12, 10) is None
+
+
+def test_struct_layouts_match_the_c_header(tmp_path):
+    """sizeof/offsetof of every ABI struct, from the C compiler, equal the
+    ctypes mirrors (catches field drift between include/mbir_hip.h and
+    _lib.py)."""
+    import ctypes
+    import shutil
+    import subprocess
+    cc = shutil.which('gcc') or shutil.which('cc')
+    if cc is None:
+        pytest.skip('no C compiler')
+    structs = {'mbir_par_geom': _lib.ParGeom, 'mbir_cone_geom': _lib.ConeGeom,
+               'mbir_vcd_params': _lib.VcdParams, 'mbir_vcd_job': _lib.VcdJob}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mbir_hip.h"',
+             'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname}.{field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ['return 0;', '}']
+    src = tmp_path / 'layout.c'
+    src.write_text('\n'.join(lines))
+    exe = tmp_path / 'layout'
+    subprocess.run([cc, '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe)],
+                   check=True)
+    got = dict(line.split() for line in subprocess.run(
+        [str(exe)], check=True, capture_output=True, text=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert int(got[f'{cname}.{field}']) == getattr(py, field).offset, (cname, field)
+
+
+def test_memory_ledger_closed_form_and_admission():
+    from mbirtorch_amd.serving import MemoryLedger, request_device_bytes
+    g = [1, 2, 4, 8, 16, 32, 64, 128, 128, 128, 128]
+    small = request_device_bytes((90, 8, 128), (128, 128, 8), g)
+    big = request_device_bytes((720, 32, 512), (512, 512, 32), g)
+    assert big['total'] > 50 * small['total']
+    # The resident part holds at least the error sinogram, recon and Hessian.
+    assert big['resident'] >= 4 * 720 * 32 * 512 + 2 * 4 * 512 * 512 * 32
+    assert request_device_bytes((90, 8, 128), (128, 128, 8), g, weighted=True)['resident'] \
+        == small['resident'] + 4 * 90 * 8 * 128
+    led = MemoryLedger(1000, margin=0.2)
+    assert led.capacity == 800
+    assert led.try_reserve(500) and not led.try_reserve(400) and led.try_reserve(300)
+    led.release(500)
+    assert led.used == 300 and led.fits_alone(800) and not led.fits_alone(801)
