@@ -41,6 +41,12 @@ def parse():
     ap.add_argument('--vcd-iters', type=int, default=10)
     ap.add_argument('--no-vcd', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--serving-requests', type=int, default=32,
+                    help='requests of the mixed serving stream (0 skips it)')
+    ap.add_argument('--serving-rate', type=float, default=40.0,
+                    help='Poisson arrival rate of the serving stream (requests/s)')
+    ap.add_argument('--serving-slots', type=int, default=8)
+    ap.add_argument('--serving-iters', type=int, default=20)
     ap.add_argument('--copy-calibration', action='store_true',
                     help='also run a plain 256 MiB device copy (the reference kernel for '
                          'scaling HBM counters) three times')
@@ -65,6 +71,62 @@ def measured_traffic(kernel):
     except (OSError, ValueError, KeyError):
         return None
     return None if entry is None else entry['bytes_per_launch']
+
+
+def serving_stream(n, rate, iters, seed=0):
+    """The mixed request stream: volume drawn from {128^2 x 8, 256^2 x 16,
+    512^2 x 32}, views uniform in 90..720 over [0, pi), detector channels =
+    width and rows = slices, seeded ellipsoid phantom + 1% noise, Poisson
+    arrivals at `rate`."""
+    import mbirtorch_amd as mb
+    from mbirtorch_amd.phantom import random_ellipsoids, add_noise
+    from mbirtorch_amd.serving import ReconRequest
+    rng = np.random.default_rng(seed)
+    sizes = [(128, 8), (256, 16), (512, 32)]
+    phantoms = {}
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    reqs = []
+    for i in range(n):
+        w, S = sizes[int(rng.integers(3))]
+        V = int(rng.integers(90, 721))
+        angles = np.linspace(0, np.pi, V, endpoint=False)
+        model = mb.ParallelBeamModel((V, S, w), angles)
+        rs = tuple(model.get_params('recon_shape'))
+        if rs not in phantoms:
+            phantoms[rs] = random_ellipsoids(rs, 10, seed=len(phantoms))
+        sino = add_noise(model.forward_project(phantoms[rs]), 0.01, seed=1000 + i)
+        reqs.append(ReconRequest(sino, angles, max_iterations=iters, seed=i,
+                                 params=dict(p=1.2, q=2.0, T=1.0), tag=i,
+                                 arrival=float(arrivals[i])))
+    return reqs
+
+
+def serving_bench(args):
+    """recons/s and p50/p99 latency of the request engine (continuous
+    batching) against the one-at-a-time plain path on the same stream."""
+    import dataclasses
+    import torch
+    from mbirtorch_amd.serving import ReconServer, run_plain, latency_summary
+    reqs = serving_stream(args.serving_requests, args.serving_rate, args.serving_iters)
+    torch.cuda.synchronize()
+    log(f'serving: plain path, {len(reqs)} requests')
+    plain = run_plain([dataclasses.replace(r) for r in reqs])
+    log('serving: request engine')
+    with ReconServer(max_active=args.serving_slots) as srv:
+        batched = srv.run_stream([dataclasses.replace(r) for r in reqs])
+    by_tag = {r.tag: r for r in plain}
+    worst = 0.0
+    for r in batched:
+        ref = by_tag[r.tag].recon
+        worst = max(worst, float(np.sqrt(np.mean((r.recon - ref) ** 2))
+                                 / max(np.sqrt(np.mean(ref ** 2)), 1e-30)))
+    b, p = latency_summary(batched), latency_summary(plain)
+    return {'requests': len(reqs), 'rate_per_s': args.serving_rate,
+            'slots': args.serving_slots, 'iterations': args.serving_iters,
+            'mix': '128^2x8 / 256^2x16 / 512^2x32, 90-720 views',
+            'batched': b, 'plain': p,
+            'speedup_recons_per_s': b['recons_per_s'] / p['recons_per_s'],
+            'max_nrmse_batched_vs_plain': worst}
 
 
 def cpu_baseline(angles, vol, sino, views):
@@ -183,6 +245,10 @@ def main():
         torch.cuda.synchronize()
         vcd_ms = (time.perf_counter() - t1) * 1e3
 
+    serving = None
+    if args.serving_requests > 0:
+        serving = serving_bench(args)
+
     cpu = None
     if not args.no_cpu:
         log('cpu baseline')
@@ -197,7 +263,7 @@ def main():
         'config': {'workload': 'parallel-beam fwd+back: 720 views [0,pi), 64x512 detector, '
                                '512x512x64 volume', 'views': V, 'det_rows': R,
                    'det_channels': C, 'volume': list(rs), 'parallelism': 'single GPU'},
-        'roofline': roofline, 'cpu_baseline': cpu,
+        'roofline': roofline, 'cpu_baseline': cpu, 'serving': serving,
         'fwd_ms': round(fwd_ms, 4), 'back_ms': round(back_ms, 4),
         'vcd_iters': None if vcd_ms is None else args.vcd_iters,
         'vcd_ms': None if vcd_ms is None else round(vcd_ms, 1),

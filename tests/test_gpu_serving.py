@@ -50,14 +50,20 @@ def test_mixed_results_equal_solo(max_active):
 
 
 def test_ledger_queues_when_memory_is_short():
+    from mbirtorch_amd import ParallelBeamModel
     from mbirtorch_amd.serving import ReconServer, request_device_bytes, MemoryPreflightError
     print('[gpu] serving ledger', flush=True)
     reqs = _requests()[:3]
-    one = max(request_device_bytes(r.sinogram.shape, (64, 64, 8),
-                                   [1, 2, 4, 8, 16, 32, 64, 128, 128, 128, 128])['total']
-              for r in reqs)
-    # Room for exactly one request at a time: the others wait in the queue.
-    with ReconServer(max_active=3, budget_bytes=int(1.5 * one / 0.85)) as srv:
+    needs = []
+    for r in reqs:
+        m = ParallelBeamModel(r.sinogram.shape, r.angles)
+        needs.append(request_device_bytes(r.sinogram.shape, m.get_params('recon_shape'),
+                                          m.get_params('granularity'))['total'])
+    lo = sorted(needs)
+    cap = lo[0] + lo[1] - 1  # every request fits alone, no two fit together
+    assert max(needs) <= cap
+    with ReconServer(max_active=3) as srv:
+        srv.ledger.capacity = cap
         for r in reqs:
             srv.submit(r)
         assert len(srv.jobs) == 1 and len(srv.queue) == 2
@@ -65,8 +71,7 @@ def test_ledger_queues_when_memory_is_short():
         while srv.pending():
             done += srv.poll(timeout_ms=50)
             assert len(srv.jobs) <= 1
-        assert len(done) == 3
+        assert sorted(d.tag for d in done) == [0, 1, 2]
+        srv.ledger.capacity = 1
         with pytest.raises(MemoryPreflightError):
-            big = _requests()[3]
-            srv.ledger.capacity = 1
-            srv.submit(big)
+            srv.submit(_requests()[3])
