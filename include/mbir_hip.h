@@ -184,6 +184,26 @@ int mbir_par_fwd_update(const float* delta, const int64_t* pixel_indices,
                         const float* view_table, const mbir_par_geom* geom,
                         const float* alpha, float* error_vcr, mbir_stream_t stream);
 
+/* bfloat16 storage (raw bits, uint16_t) of the channel-major error sinogram
+ * and weights, with float32 arithmetic: the VCD back projection (coeff_power
+ * 1) and the binned forward line search / update.  Same contracts as the
+ * float32 entry points (line-search partials: mbir_par_fwd_partials_count(
+ * geom, 0)); error_vcr is rounded to nearest even on update.  No reference
+ * counterpart: the storage format of the mixed-precision path. */
+int mbir_par_back_bf16(const uint16_t* sino_vcr, const uint16_t* weights_vcr,
+                       const int64_t* pixel_indices, int32_t num_pixels,
+                       const float* view_table, const mbir_par_geom* geom,
+                       int32_t num_splits, float* partials, mbir_stream_t stream);
+int mbir_par_fwd_linesearch_bf16(const float* delta, const int64_t* pixel_indices,
+                                 int32_t num_pixels, const float* view_table,
+                                 const mbir_par_geom* geom, const uint16_t* error_vcr,
+                                 const uint16_t* weights_vcr, double* partials,
+                                 mbir_stream_t stream);
+int mbir_par_fwd_update_bf16(const float* delta, const int64_t* pixel_indices,
+                             int32_t num_pixels, const float* view_table,
+                             const mbir_par_geom* geom, const float* alpha,
+                             uint16_t* error_vcr, mbir_stream_t stream);
+
 /* ---- cone-beam projectors ---------------------------------------------- */
 
 /* Sparse cone forward projection (detector-side vertical fan, then the
@@ -286,10 +306,28 @@ int mbir_vcd_apply(float* flat_recon, const int64_t* pixel_indices,
                    const float* alpha, double* acc, double* scratch,
                    mbir_stream_t stream);
 
+/* bfloat16-storage cone variants (see the parallel ones). */
+int mbir_cone_back_bf16(const uint16_t* sino_vcr, const uint16_t* weights_vcr,
+                        const int64_t* pixel_indices, int32_t num_pixels,
+                        const float* view_table, const mbir_cone_geom* geom,
+                        int32_t num_splits, float* partials, mbir_stream_t stream);
+int mbir_cone_fwd_linesearch_bf16(const float* delta, const int64_t* pixel_indices,
+                                  int32_t num_pixels, const float* view_table,
+                                  const mbir_cone_geom* geom, const uint16_t* error_vcr,
+                                  const uint16_t* weights_vcr, double* partials,
+                                  mbir_stream_t stream);
+int mbir_cone_fwd_update_bf16(const float* delta, const int64_t* pixel_indices,
+                              int32_t num_pixels, const float* view_table,
+                              const mbir_cone_geom* geom, const float* alpha,
+                              uint16_t* error_vcr, mbir_stream_t stream);
+
 /* ---- native VCD schedule runner ---------------------------------------- */
 
 #define MBIR_GEOMETRY_PARALLEL 0
 #define MBIR_GEOMETRY_CONE 1
+
+#define MBIR_DTYPE_F32 0
+#define MBIR_DTYPE_BF16 1
 
 /* Everything one VCD reconstruction needs on the device, plus host tables of
  * its partition subsets (subset id -> device index array, size, optional
@@ -304,7 +342,7 @@ typedef struct {
   int32_t geometry;                 /* MBIR_GEOMETRY_* */
   int32_t positivity;               /* clip x + alpha*delta at zero */
   int32_t num_subsets;              /* entries of the subset tables */
-  int32_t pad0;
+  int32_t sino_dtype;               /* MBIR_DTYPE_*: storage of error_vcr, weights_vcr */
   const mbir_par_geom* par_geom;    /* host pointer (parallel) */
   const mbir_cone_geom* cone_geom;  /* host pointer (cone) */
   const float* view_table;
@@ -385,6 +423,14 @@ int32_t mbir_reduce_blocks(int64_t n);
  * (tomography_model.py:2323-2357). */
 int mbir_sumsq_f64(const float* a, const float* w, int64_t n, double* scratch,
                    double* out, mbir_stream_t stream);
+
+/* mbir_sumsq_f64 over bfloat16 arrays. */
+int mbir_sumsq_bf16(const uint16_t* a, const uint16_t* w, int64_t n, double* scratch,
+                    double* out, mbir_stream_t stream);
+
+/* Storage conversion float32 <-> bfloat16 (round to nearest even). */
+int mbir_f32_to_bf16(const float* in, uint16_t* out, int64_t n, mbir_stream_t stream);
+int mbir_bf16_to_f32(const uint16_t* in, float* out, int64_t n, mbir_stream_t stream);
 
 /* out[0] = sum |a|, FP64 (image_ell1, _memory_ledger.py:134-163). */
 int mbir_sumabs_f64(const float* a, int64_t n, double* scratch, double* out,

@@ -66,12 +66,14 @@ class VcdParams(ctypes.Structure):
 
 MBIR_GEOMETRY_PARALLEL = 0
 MBIR_GEOMETRY_CONE = 1
+MBIR_DTYPE_F32 = 0
+MBIR_DTYPE_BF16 = 1
 
 
 class VcdJob(ctypes.Structure):
     """mbir_vcd_job: one reconstruction's device state and subset tables."""
     _fields_ = [('geometry', ctypes.c_int32), ('positivity', ctypes.c_int32),
-                ('num_subsets', ctypes.c_int32), ('pad0', ctypes.c_int32),
+                ('num_subsets', ctypes.c_int32), ('sino_dtype', ctypes.c_int32),
                 ('par_geom', ctypes.POINTER(ParGeom)), ('cone_geom', ctypes.POINTER(ConeGeom)),
                 ('view_table', ctypes.c_void_p), ('prm', VcdParams),
                 ('flat_recon', ctypes.c_void_p), ('error_vcr', ctypes.c_void_p),
@@ -119,6 +121,15 @@ SIGNATURES = {
     'mbir_vcd_alpha': (ctypes.c_int, [_P, _I32, _P, _I32, _VP, _P, _P, _P]),
     'mbir_vcd_positivity': (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P]),
     'mbir_vcd_apply_scratch': (ctypes.c_int64, [_I32, _I32]),
+    'mbir_par_back_bf16': (ctypes.c_int, [_P, _P, _P, _I32, _P, _GP, _I32, _P, _P]),
+    'mbir_par_fwd_linesearch_bf16': (ctypes.c_int, [_P, _P, _I32, _P, _GP, _P, _P, _P, _P]),
+    'mbir_par_fwd_update_bf16': (ctypes.c_int, [_P, _P, _I32, _P, _GP, _P, _P, _P]),
+    'mbir_cone_back_bf16': (ctypes.c_int, [_P, _P, _P, _I32, _P, _CP, _I32, _P, _P]),
+    'mbir_cone_fwd_linesearch_bf16': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _P, _P, _P, _P]),
+    'mbir_cone_fwd_update_bf16': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _P, _P, _P]),
+    'mbir_sumsq_bf16': (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    'mbir_f32_to_bf16': (ctypes.c_int, [_P, _P, _I64, _P]),
+    'mbir_bf16_to_f32': (ctypes.c_int, [_P, _P, _I64, _P]),
     'mbir_vcd_run': (ctypes.c_int, [ctypes.POINTER(VcdJob), _P, _I32, _P, _I32, _P]),
     'mbir_server_create': (ctypes.c_void_p, [_I32]),
     'mbir_server_submit': (ctypes.c_int, [_P, ctypes.POINTER(VcdJob), _P, _P, _I32, _P, _I64]),

@@ -61,6 +61,10 @@ def _defaults():
         'verbose': Param(1, False),
         'max_alpha': Param(1.5, False),
         'use_ror_mask': Param(True, False),
+        # storage of the VCD error sinogram and weights on the device:
+        # 'float32', or 'bfloat16' (float32 arithmetic, FP64 reductions).
+        # Not a reference parameter.
+        'sinogram_storage': Param('float32', False),
     }
     return d
 

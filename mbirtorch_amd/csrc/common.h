@@ -33,6 +33,26 @@ struct ParView {
   float pad0, pad1;
 };
 
+// Sinogram storage: float32, or bfloat16 kept as its 16 raw bits (the high
+// half of the float32 pattern).  Loads widen exactly; stores round to
+// nearest even (finite values).
+typedef uint16_t bf16_bits;
+
+__device__ __forceinline__ float bf16_to_f32(bf16_bits h) {
+  return __uint_as_float((uint32_t)h << 16);
+}
+
+__device__ __forceinline__ bf16_bits f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (bf16_bits)(u >> 16);
+}
+
+__device__ __forceinline__ float ld_s(const float* p) { return *p; }
+__device__ __forceinline__ float ld_s(const bf16_bits* p) { return bf16_to_f32(*p); }
+__device__ __forceinline__ void st_s(float* p, float v) { *p = v; }
+__device__ __forceinline__ void st_s(bf16_bits* p, float v) { *p = f32_to_bf16(v); }
+
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, MBIR_WAVE);

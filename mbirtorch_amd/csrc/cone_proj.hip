@@ -94,9 +94,9 @@ __device__ __forceinline__ void cone_htaps(const ConePixel& q, const mbir_cone_g
 
 // ---------------------------------------------------------------------------
 // Back projection.  LDS record per (view, pixel): base, NH weights, m0, Wr, mag.
-template <int RW, int NH, int NV, int CP, bool WEIGHTED>
+template <int RW, int NH, int NV, int CP, bool WEIGHTED, typename TS>
 __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
-    const float* __restrict__ sino, const float* __restrict__ wts,
+    const TS* __restrict__ sino, const TS* __restrict__ wts,
     const int64_t* __restrict__ pix, int n_pix, const ConeView* __restrict__ views,
     mbir_cone_geom g, int views_per_split, float* __restrict__ out) {
   constexpr int SLOTS = MBIR_WAVE / RW;
@@ -177,8 +177,8 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
             for (int th = 0; th < NH; ++th) {
               const float ah = s_rec[(1 + th) * STRIDE + qd];
               const int n = min(max(base + th, 0), C - 1);
-              float e = sino[vofs + (size_t)n * R + mcl];
-              if (WEIGHTED) e = e * wts[vofs + (size_t)n * R + mcl];
+              float e = ld_s(sino + vofs + (size_t)n * R + mcl);
+              if (WEIGHTED) e = e * ld_s(wts + vofs + (size_t)n * R + mcl);
               det = fmaf(ah, e, det);
             }
             sum = fmaf(Av, det, sum);
@@ -201,19 +201,19 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
 // Forward projection into a per-view row-band tile.
 enum ConeFwdMode { CF_WRITE_VCR = 0, CF_WRITE_VRC = 1, CF_LINESEARCH = 2, CF_UPDATE = 3 };
 
-template <int RC, int NH, int NK, int MODE, bool WEIGHTED>
+template <int RC, int NH, int NK, int MODE, bool WEIGHTED, typename TS>
 __global__ __launch_bounds__(kCBlock) void cone_fwd_tile_kernel(
     const float* __restrict__ values, const int64_t* __restrict__ pix, int n_pix,
-    const ConeView* __restrict__ views, mbir_cone_geom g, float* __restrict__ sino,
-    const float* __restrict__ wts, const float* __restrict__ alpha,
+    const ConeView* __restrict__ views, mbir_cone_geom g, TS* __restrict__ sino,
+    const TS* __restrict__ wts, const float* __restrict__ alpha,
     double* __restrict__ partials) {
-  constexpr int TS = RC + 1;
+  constexpr int TSTR = RC + 1;
   constexpr int SLOTS = MBIR_WAVE / RC;
   constexpr int REC = NH + 4;
   extern __shared__ float smem[];
   const int C = g.num_channels, R = g.num_det_rows, S = g.num_slices;
-  float* tile = smem;                      // C x TS
-  float* s_rec = tile + (size_t)C * TS;    // REC x kCBlock
+  float* tile = smem;                      // C x TSTR
+  float* s_rec = tile + (size_t)C * TSTR;  // REC x kCBlock
 
   const int v = blockIdx.x, m_lo = blockIdx.y * RC;
   const int tid = threadIdx.x;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_tile_kernel(
   const ConeView vw = views[v];
   const float sdd = g.source_detector_dist;
 
-  for (int i = tid; i < C * TS; i += kCBlock) tile[i] = 0.0f;
+  for (int i = tid; i < C * TSTR; i += kCBlock) tile[i] = 0.0f;
 
   int64_t p_next = tid < n_pix ? pix[tid] : 0;
   for (int pc = 0; pc < n_pix; pc += kCBlock) {
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_tile_kernel(
 #pragma unroll
       for (int th = 0; th < NH; ++th) {
         const int n = min(max(b + th, 0), C - 1);
-        atomicAdd(&tile[n * TS + rl], s_rec[(1 + th) * kCBlock + j] * det);
+        atomicAdd(&tile[n * TSTR + rl], s_rec[(1 + th) * kCBlock + j] * det);
       }
     }
   }
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_tile_kernel(
   if (MODE == CF_WRITE_VRC) {
     for (int i = tid; i < C * RC; i += kCBlock) {
       const int rr = i / C, c = i - rr * C;
-      if (m_lo + rr < R) sino[((size_t)v * R + m_lo + rr) * C + c] = tile[c * TS + rr];
+      if (m_lo + rr < R) st_s(sino + ((size_t)v * R + m_lo + rr) * C + c, tile[c * TSTR + rr]);
     }
   } else {
     double lin = 0.0, quad = 0.0;
@@ -298,13 +298,13 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_tile_kernel(
       const int c = i / RC, rr = i - c * RC;
       if (m_lo + rr >= R) continue;
       const size_t o = ((size_t)v * C + c) * R + m_lo + rr;
-      const float t = tile[c * TS + rr];
+      const float t = tile[c * TSTR + rr];
       if (MODE == CF_WRITE_VCR) {
-        sino[o] = t;
+        st_s(sino + o, t);
       } else if (MODE == CF_LINESEARCH) {
-        const float e = sino[o];
+        const float e = ld_s(sino + o);
         if (WEIGHTED) {
-          const float w = wts[o];
+          const float w = ld_s(wts + o);
           lin += (double)((w * e) * t);
           quad += (double)((t * t) * w);
         } else {
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_tile_kernel(
           quad += (double)(t * t);
         }
       } else {
-        sino[o] = sino[o] - al * t;
+        st_s(sino + o, ld_s(sino + o) - al * t);
       }
     }
     if (MODE == CF_LINESEARCH) {
@@ -359,112 +359,140 @@ static inline int cone_fwd_rc(const mbir_cone_geom* g) {
   return rc;
 }
 
+// Sinogram storage: float32, or bfloat16 bits (back with cp == 1, forward
+// line search / update).  Tap-window templates: 3 taps, or 9 (the widest
+// window; the trapezoid weights vanish outside the reference's narrower
+// windows, so a wider window gives the same sums).
 template <int RW, int NH, int NV>
-static int cone_back_launch3(const float* sino, const float* w, const int64_t* pix, int P,
+static int cone_back_launch3(const void* sino, const void* w, const int64_t* pix, int P,
                              const ConeView* views, const mbir_cone_geom& g, int cp, int ns,
-                             float* out, hipStream_t s) {
+                             float* out, hipStream_t s, bool bf16) {
   const int pwg = cone_pix_per_wg(RW);
   const int vps = (g.num_views + ns - 1) / ns;
   dim3 grid((P + pwg - 1) / pwg, ns, (g.num_slices + RW - 1) / RW);
-  if (cp == 1) {
-    if (w) cone_back_kernel<RW, NH, NV, 1, true><<<grid, kCBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
-    else cone_back_kernel<RW, NH, NV, 1, false><<<grid, kCBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
+  const float* sf = static_cast<const float*>(sino);
+  const float* wf = static_cast<const float*>(w);
+  const bf16_bits* sh = static_cast<const bf16_bits*>(sino);
+  const bf16_bits* wh = static_cast<const bf16_bits*>(w);
+  if (bf16) {
+    if (w) cone_back_kernel<RW, NH, NV, 1, true, bf16_bits><<<grid, kCBlock, 0, s>>>(sh, wh, pix, P, views, g, vps, out);
+    else cone_back_kernel<RW, NH, NV, 1, false, bf16_bits><<<grid, kCBlock, 0, s>>>(sh, wh, pix, P, views, g, vps, out);
+  } else if (cp == 1) {
+    if (w) cone_back_kernel<RW, NH, NV, 1, true, float><<<grid, kCBlock, 0, s>>>(sf, wf, pix, P, views, g, vps, out);
+    else cone_back_kernel<RW, NH, NV, 1, false, float><<<grid, kCBlock, 0, s>>>(sf, wf, pix, P, views, g, vps, out);
   } else {
-    if (w) cone_back_kernel<RW, NH, NV, 2, true><<<grid, kCBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
-    else cone_back_kernel<RW, NH, NV, 2, false><<<grid, kCBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
+    if (w) cone_back_kernel<RW, NH, NV, 2, true, float><<<grid, kCBlock, 0, s>>>(sf, wf, pix, P, views, g, vps, out);
+    else cone_back_kernel<RW, NH, NV, 2, false, float><<<grid, kCBlock, 0, s>>>(sf, wf, pix, P, views, g, vps, out);
   }
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }
 
 template <int NT>
-static int cone_back_taps(const float* sino, const float* w, const int64_t* pix, int P,
+static int cone_back_taps(const void* sino, const void* w, const int64_t* pix, int P,
                           const ConeView* views, const mbir_cone_geom& g, int cp, int ns,
-                          float* out, hipStream_t s) {
+                          float* out, hipStream_t s, bool bf16) {
   switch (cone_rw(g.num_slices)) {
-    case 64: return cone_back_launch3<64, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    case 32: return cone_back_launch3<32, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    case 16: return cone_back_launch3<16, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    case 8: return cone_back_launch3<8, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    case 4: return cone_back_launch3<4, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    case 2: return cone_back_launch3<2, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    default: return cone_back_launch3<1, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
+    case 64: return cone_back_launch3<64, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    case 32: return cone_back_launch3<32, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    case 16: return cone_back_launch3<16, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    case 8: return cone_back_launch3<8, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    case 4: return cone_back_launch3<4, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    case 2: return cone_back_launch3<2, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    default: return cone_back_launch3<1, NT, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
   }
 }
 
+struct ConeFwdArgs {
+  int mode;
+  const float* values;
+  const int64_t* pix;
+  int P;
+  const ConeView* views;
+  mbir_cone_geom g;
+  void* sino;
+  const void* w;
+  const float* alpha;
+  double* partials;
+  hipStream_t s;
+  bool bf16;
+};
+
+template <int RC, int NH, int NK, int MODE, bool WEIGHTED, typename TS>
+static void cone_fwd_kernel_launch(const ConeFwdArgs& a) {
+  dim3 grid(a.g.num_views, (a.g.num_det_rows + RC - 1) / RC);
+  const size_t lds = cone_fwd_lds(a.g.num_channels, RC, NH);
+  cone_fwd_tile_kernel<RC, NH, NK, MODE, WEIGHTED, TS><<<grid, kCBlock, lds, a.s>>>(
+      a.values, a.pix, a.P, a.views, a.g, static_cast<TS*>(a.sino),
+      static_cast<const TS*>(a.w), a.alpha, a.partials);
+}
+
 template <int RC, int NH, int NK>
-static int cone_fwd_launch3(int mode, const float* values, const int64_t* pix, int P,
-                            const ConeView* views, const mbir_cone_geom& g, float* sino,
-                            const float* w, const float* alpha, double* partials,
-                            hipStream_t s) {
-  dim3 grid(g.num_views, (g.num_det_rows + RC - 1) / RC);
-  const size_t lds = cone_fwd_lds(g.num_channels, RC, NH);
-  switch (mode) {
-    case CF_WRITE_VCR:
-      cone_fwd_tile_kernel<RC, NH, NK, CF_WRITE_VCR, false><<<grid, kCBlock, lds, s>>>(
-          values, pix, P, views, g, sino, w, alpha, partials);
-      break;
-    case CF_WRITE_VRC:
-      cone_fwd_tile_kernel<RC, NH, NK, CF_WRITE_VRC, false><<<grid, kCBlock, lds, s>>>(
-          values, pix, P, views, g, sino, w, alpha, partials);
-      break;
-    case CF_LINESEARCH:
-      if (w)
-        cone_fwd_tile_kernel<RC, NH, NK, CF_LINESEARCH, true><<<grid, kCBlock, lds, s>>>(
-            values, pix, P, views, g, sino, w, alpha, partials);
-      else
-        cone_fwd_tile_kernel<RC, NH, NK, CF_LINESEARCH, false><<<grid, kCBlock, lds, s>>>(
-            values, pix, P, views, g, sino, w, alpha, partials);
-      break;
-    default:
-      cone_fwd_tile_kernel<RC, NH, NK, CF_UPDATE, false><<<grid, kCBlock, lds, s>>>(
-          values, pix, P, views, g, sino, w, alpha, partials);
+static int cone_fwd_launch3(const ConeFwdArgs& a) {
+  if (a.bf16) {
+    if (a.mode == CF_LINESEARCH) {
+      if (a.w) cone_fwd_kernel_launch<RC, NH, NK, CF_LINESEARCH, true, bf16_bits>(a);
+      else cone_fwd_kernel_launch<RC, NH, NK, CF_LINESEARCH, false, bf16_bits>(a);
+    } else {
+      cone_fwd_kernel_launch<RC, NH, NK, CF_UPDATE, false, bf16_bits>(a);
+    }
+  } else {
+    switch (a.mode) {
+      case CF_WRITE_VCR: cone_fwd_kernel_launch<RC, NH, NK, CF_WRITE_VCR, false, float>(a); break;
+      case CF_WRITE_VRC: cone_fwd_kernel_launch<RC, NH, NK, CF_WRITE_VRC, false, float>(a); break;
+      case CF_LINESEARCH:
+        if (a.w) cone_fwd_kernel_launch<RC, NH, NK, CF_LINESEARCH, true, float>(a);
+        else cone_fwd_kernel_launch<RC, NH, NK, CF_LINESEARCH, false, float>(a);
+        break;
+      default: cone_fwd_kernel_launch<RC, NH, NK, CF_UPDATE, false, float>(a);
+    }
   }
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }
 
 template <int NH, int NK>
-static int cone_fwd_rc_sw(int mode, const float* values, const int64_t* pix, int P,
-                          const ConeView* views, const mbir_cone_geom& g, float* sino,
-                          const float* w, const float* alpha, double* partials, hipStream_t s) {
-  switch (cone_fwd_rc(&g)) {
-    case 64: return cone_fwd_launch3<64, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 32: return cone_fwd_launch3<32, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 16: return cone_fwd_launch3<16, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 8: return cone_fwd_launch3<8, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 4: return cone_fwd_launch3<4, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 2: return cone_fwd_launch3<2, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    default: return cone_fwd_launch3<1, NH, NK>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
+static int cone_fwd_rc_sw(const ConeFwdArgs& a) {
+  switch (cone_fwd_rc(&a.g)) {
+    case 64: return cone_fwd_launch3<64, NH, NK>(a);
+    case 32: return cone_fwd_launch3<32, NH, NK>(a);
+    case 16: return cone_fwd_launch3<16, NH, NK>(a);
+    case 8: return cone_fwd_launch3<8, NH, NK>(a);
+    case 4: return cone_fwd_launch3<4, NH, NK>(a);
+    case 2: return cone_fwd_launch3<2, NH, NK>(a);
+    default: return cone_fwd_launch3<1, NH, NK>(a);
   }
 }
 
-template <int NH>
-static int cone_fwd_bp(int mode, const float* values, const int64_t* pix, int P,
-                       const ConeView* views, const mbir_cone_geom& g, float* sino,
-                       const float* w, const float* alpha, double* partials, hipStream_t s) {
-  switch (2 * g.bp_psf_radius + 1) {
-    case 3: return cone_fwd_rc_sw<NH, 3>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 5: return cone_fwd_rc_sw<NH, 5>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 7: return cone_fwd_rc_sw<NH, 7>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    default: return cone_fwd_rc_sw<NH, 9>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-  }
+static int cone_fwd(const ConeFwdArgs& a) {
+  if (a.bf16 && (a.mode == CF_WRITE_VCR || a.mode == CF_WRITE_VRC)) return MBIR_ERR_ARG;
+  const bool nh3 = 2 * a.g.psf_radius + 1 <= 3, nk3 = 2 * a.g.bp_psf_radius + 1 <= 3;
+  if (nh3) return nk3 ? cone_fwd_rc_sw<3, 3>(a) : cone_fwd_rc_sw<3, 9>(a);
+  return nk3 ? cone_fwd_rc_sw<9, 3>(a) : cone_fwd_rc_sw<9, 9>(a);
 }
 
-static int cone_fwd(int mode, const float* values, const int64_t* pix, int P,
-                    const ConeView* views, const mbir_cone_geom& g, float* sino, const float* w,
-                    const float* alpha, double* partials, hipStream_t s) {
-  switch (2 * g.psf_radius + 1) {
-    case 3: return cone_fwd_bp<3>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 5: return cone_fwd_bp<5>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    case 7: return cone_fwd_bp<7>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-    default: return cone_fwd_bp<9>(mode, values, pix, P, views, g, sino, w, alpha, partials, s);
-  }
-}
+static int cone_back_any(const void* sino, const void* w, const int64_t* pix, int32_t P,
+                         const float* view_table, const mbir_cone_geom* g, int32_t cp, int32_t ns,
+                         float* out, mbir_stream_t stream, bool bf16);
 
 }  // namespace mbir
 
 using namespace mbir;
+
+int mbir::cone_back_any(const void* sino, const void* w, const int64_t* pix, int32_t P,
+                          const float* view_table, const mbir_cone_geom* g, int32_t cp,
+                          int32_t ns, float* out, mbir_stream_t stream, bool bf16) {
+  int st = check_cone(g);
+  if (st) return st;
+  if (!sino || !pix || !view_table || !out || P < 0) return MBIR_ERR_ARG;
+  if ((cp != 1 && cp != 2) || ns < 1 || ns > g->num_views) return MBIR_ERR_ARG;
+  if (P == 0) return MBIR_OK;
+  const ConeView* views = reinterpret_cast<const ConeView*>(view_table);
+  hipStream_t s = (hipStream_t)stream;
+  if (2 * g->psf_radius + 1 <= 3)
+    return cone_back_taps<3>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
+  return cone_back_taps<9>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
+}
 
 extern "C" {
 
@@ -483,19 +511,13 @@ int32_t mbir_cone_back_splits(const mbir_cone_geom* g, int32_t P) {
 int mbir_cone_back(const float* sino, const float* w, const int64_t* pix, int32_t P,
                    const float* view_table, const mbir_cone_geom* g, int32_t cp, int32_t ns,
                    float* out, mbir_stream_t stream) {
-  int st = check_cone(g);
-  if (st) return st;
-  if (!sino || !pix || !view_table || !out || P < 0) return MBIR_ERR_ARG;
-  if ((cp != 1 && cp != 2) || ns < 1 || ns > g->num_views) return MBIR_ERR_ARG;
-  if (P == 0) return MBIR_OK;
-  const ConeView* views = reinterpret_cast<const ConeView*>(view_table);
-  hipStream_t s = (hipStream_t)stream;
-  switch (2 * g->psf_radius + 1) {
-    case 3: return cone_back_taps<3>(sino, w, pix, P, views, *g, cp, ns, out, s);
-    case 5: return cone_back_taps<5>(sino, w, pix, P, views, *g, cp, ns, out, s);
-    case 7: return cone_back_taps<7>(sino, w, pix, P, views, *g, cp, ns, out, s);
-    default: return cone_back_taps<9>(sino, w, pix, P, views, *g, cp, ns, out, s);
-  }
+  return cone_back_any(sino, w, pix, P, view_table, g, cp, ns, out, stream, false);
+}
+
+int mbir_cone_back_bf16(const uint16_t* sino, const uint16_t* w, const int64_t* pix, int32_t P,
+                        const float* view_table, const mbir_cone_geom* g, int32_t ns, float* out,
+                        mbir_stream_t stream) {
+  return cone_back_any(sino, w, pix, P, view_table, g, 1, ns, out, stream, true);
 }
 
 int mbir_cone_forward(const float* values, const int64_t* pix, int32_t P,
@@ -505,9 +527,10 @@ int mbir_cone_forward(const float* values, const int64_t* pix, int32_t P,
   if (st) return st;
   if (!values || !pix || !view_table || !out || P < 0) return MBIR_ERR_ARG;
   if (layout != MBIR_LAYOUT_VRC && layout != MBIR_LAYOUT_VCR) return MBIR_ERR_ARG;
-  return cone_fwd(layout == MBIR_LAYOUT_VRC ? CF_WRITE_VRC : CF_WRITE_VCR, values, pix, P,
-                  reinterpret_cast<const ConeView*>(view_table), *g, out, nullptr, nullptr,
-                  nullptr, (hipStream_t)stream);
+  ConeFwdArgs a{layout == MBIR_LAYOUT_VRC ? CF_WRITE_VRC : CF_WRITE_VCR, values, pix, P,
+                reinterpret_cast<const ConeView*>(view_table), *g, out, nullptr, nullptr,
+                nullptr, (hipStream_t)stream, false};
+  return cone_fwd(a);
 }
 
 int32_t mbir_cone_fwd_partials_count(const mbir_cone_geom* g) {
@@ -516,26 +539,52 @@ int32_t mbir_cone_fwd_partials_count(const mbir_cone_geom* g) {
   return g->num_views * ((g->num_det_rows + rc - 1) / rc);
 }
 
+static int cone_ls_any(const float* delta, const int64_t* pix, int32_t P, const float* view_table,
+                       const mbir_cone_geom* g, const void* error_vcr, const void* w,
+                       double* partials, mbir_stream_t stream, bool bf16) {
+  int st = check_cone(g);
+  if (st) return st;
+  if (!delta || !pix || !view_table || !error_vcr || !partials || P < 0) return MBIR_ERR_ARG;
+  ConeFwdArgs a{CF_LINESEARCH, delta, pix, P, reinterpret_cast<const ConeView*>(view_table), *g,
+                const_cast<void*>(error_vcr), w, nullptr, partials, (hipStream_t)stream, bf16};
+  return cone_fwd(a);
+}
+
+static int cone_update_any(const float* delta, const int64_t* pix, int32_t P,
+                           const float* view_table, const mbir_cone_geom* g, const float* alpha,
+                           void* error_vcr, mbir_stream_t stream, bool bf16) {
+  int st = check_cone(g);
+  if (st) return st;
+  if (!delta || !pix || !view_table || !error_vcr || !alpha || P < 0) return MBIR_ERR_ARG;
+  ConeFwdArgs a{CF_UPDATE, delta, pix, P, reinterpret_cast<const ConeView*>(view_table), *g,
+                error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream, bf16};
+  return cone_fwd(a);
+}
+
 int mbir_cone_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
                              const float* view_table, const mbir_cone_geom* g,
                              const float* error_vcr, const float* w, double* partials,
                              mbir_stream_t stream) {
-  int st = check_cone(g);
-  if (st) return st;
-  if (!delta || !pix || !view_table || !error_vcr || !partials || P < 0) return MBIR_ERR_ARG;
-  return cone_fwd(CF_LINESEARCH, delta, pix, P, reinterpret_cast<const ConeView*>(view_table),
-                  *g, const_cast<float*>(error_vcr), w, nullptr, partials,
-                  (hipStream_t)stream);
+  return cone_ls_any(delta, pix, P, view_table, g, error_vcr, w, partials, stream, false);
+}
+
+int mbir_cone_fwd_linesearch_bf16(const float* delta, const int64_t* pix, int32_t P,
+                                  const float* view_table, const mbir_cone_geom* g,
+                                  const uint16_t* error_vcr, const uint16_t* w, double* partials,
+                                  mbir_stream_t stream) {
+  return cone_ls_any(delta, pix, P, view_table, g, error_vcr, w, partials, stream, true);
 }
 
 int mbir_cone_fwd_update(const float* delta, const int64_t* pix, int32_t P,
                          const float* view_table, const mbir_cone_geom* g, const float* alpha,
                          float* error_vcr, mbir_stream_t stream) {
-  int st = check_cone(g);
-  if (st) return st;
-  if (!delta || !pix || !view_table || !error_vcr || !alpha || P < 0) return MBIR_ERR_ARG;
-  return cone_fwd(CF_UPDATE, delta, pix, P, reinterpret_cast<const ConeView*>(view_table), *g,
-                  error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream);
+  return cone_update_any(delta, pix, P, view_table, g, alpha, error_vcr, stream, false);
+}
+
+int mbir_cone_fwd_update_bf16(const float* delta, const int64_t* pix, int32_t P,
+                              const float* view_table, const mbir_cone_geom* g,
+                              const float* alpha, uint16_t* error_vcr, mbir_stream_t stream) {
+  return cone_update_any(delta, pix, P, view_table, g, alpha, error_vcr, stream, true);
 }
 
 }  // extern "C"

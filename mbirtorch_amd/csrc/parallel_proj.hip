@@ -91,9 +91,9 @@ __device__ __forceinline__ bool par_tap_of(int row, int col, int n, const ParVie
 
 // ---------------------------------------------------------------------------
 // Back projection.
-template <int RW, int NT, int CP, bool WEIGHTED>
+template <int RW, int NT, int CP, bool WEIGHTED, typename TS>
 __global__ __launch_bounds__(kBlock) void par_back_kernel(
-    const float* __restrict__ sino, const float* __restrict__ wts,
+    const TS* __restrict__ sino, const TS* __restrict__ wts,
     const int64_t* __restrict__ pix, int n_pix, const ParView* __restrict__ views,
     mbir_par_geom g, int views_per_split, float* __restrict__ out) {
   constexpr int SLOTS = MBIR_WAVE / RW;
@@ -139,8 +139,8 @@ __global__ __launch_bounds__(kBlock) void par_back_kernel(
     if (r_ok) {
       for (int b = 0; b < nvb; ++b) {
         const size_t vofs = (size_t)(vb0 + b) * C * R + r;
-        const float* sv = sino + vofs;
-        const float* wv = WEIGHTED ? wts + vofs : nullptr;
+        const TS* sv = sino + vofs;
+        const TS* wv = WEIGHTED ? wts + vofs : nullptr;
         if constexpr (SLOTS == 1 && NT == 2) {
           // One pixel slot per wave: the ACC tap bases are wave-uniform.  When
           // they span at most ACC channels (neighbouring pixels of an image
@@ -166,8 +166,8 @@ __global__ __launch_bounds__(kBlock) void par_back_kernel(
               win[m] = 0.0f;
               if (m < len) {
                 const int n = min(max(lo + m, 0), C - 1);
-                float e = sv[(size_t)n * R];
-                if (WEIGHTED) e = e * wv[(size_t)n * R];
+                float e = ld_s(sv + (size_t)n * R);
+                if (WEIGHTED) e = e * ld_s(wv + (size_t)n * R);
                 win[m] = e;
               }
             }
@@ -188,8 +188,8 @@ __global__ __launch_bounds__(kBlock) void par_back_kernel(
           for (int t = 0; t < NT; ++t) {
             const float a = s_a[t * VB * PIX_WG + q];
             const int n = min(max(base + t, 0), C - 1);
-            float e = sv[(size_t)n * R];
-            if (WEIGHTED) e = e * wv[(size_t)n * R];
+            float e = ld_s(sv + (size_t)n * R);
+            if (WEIGHTED) e = e * ld_s(wv + (size_t)n * R);
             acc[k] = fmaf(a, e, acc[k]);
           }
         }
@@ -262,18 +262,52 @@ __device__ __forceinline__ void store_vr(float* p, const float (&x)[VR]) {
   }
 }
 
+// bfloat16 rows: two values per 32-bit word, element 0 in the low half.
+template <int VR>
+__device__ __forceinline__ void load_vr(const bf16_bits* p, float (&x)[VR]) {
+  if constexpr (VR % 4 == 0) {
+#pragma unroll
+    for (int h = 0; h < VR; h += 4) {
+      const uint2 t = *reinterpret_cast<const uint2*>(p + h);
+      x[h] = __uint_as_float(t.x << 16);
+      x[h + 1] = __uint_as_float(t.x & 0xffff0000u);
+      x[h + 2] = __uint_as_float(t.y << 16);
+      x[h + 3] = __uint_as_float(t.y & 0xffff0000u);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < VR; ++q) x[q] = bf16_to_f32(p[q]);
+  }
+}
+
+template <int VR>
+__device__ __forceinline__ void store_vr(bf16_bits* p, const float (&x)[VR]) {
+  if constexpr (VR % 4 == 0) {
+#pragma unroll
+    for (int h = 0; h < VR; h += 4) {
+      uint2 t;
+      t.x = (uint32_t)f32_to_bf16(x[h]) | ((uint32_t)f32_to_bf16(x[h + 1]) << 16);
+      t.y = (uint32_t)f32_to_bf16(x[h + 2]) | ((uint32_t)f32_to_bf16(x[h + 3]) << 16);
+      *reinterpret_cast<uint2*>(p + h) = t;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < VR; ++q) p[q] = f32_to_bf16(x[q]);
+  }
+}
+
 enum FwdMode { FWD_WRITE_VCR = 0, FWD_WRITE_VRC = 1, FWD_LINESEARCH = 2, FWD_UPDATE = 3 };
 
 // Channel-major epilogue of one lane: sinogram entries o .. o+VR-1 hold
 // (v, n, rr .. rr+VR-1).  Line search: FP64 sums of (w*e)*t and (t*t)*w;
 // update: e -= alpha*t.
-template <int VR, int MODE, bool WEIGHTED>
+template <int VR, int MODE, bool WEIGHTED, typename TS>
 __device__ __forceinline__ void fwd_epilogue_vcr(const float (&t)[VR], size_t o,
-                                                 float* __restrict__ sino,
-                                                 const float* __restrict__ wts, float al,
+                                                 TS* __restrict__ sino,
+                                                 const TS* __restrict__ wts, float al,
                                                  double& lin, double& quad) {
   if (MODE == FWD_WRITE_VCR) {
-    store_vr<VR>(sino + o, t);
+    if constexpr (sizeof(TS) == sizeof(float)) store_vr<VR>(sino + o, t);
   } else if (MODE == FWD_LINESEARCH) {
     float e[VR];
     load_vr<VR>(sino + o, e);
@@ -554,11 +588,11 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp) {
   return prefix + x - v;
 }
 
-template <int LPC, int VR, bool TWO, int MODE, bool WEIGHTED>
+template <int LPC, int VR, bool TWO, int MODE, bool WEIGHTED, typename TS>
 __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
     const float* __restrict__ values, const int64_t* __restrict__ pix, int n_pix,
-    const ParView* __restrict__ views, mbir_par_geom g, float* __restrict__ sino,
-    const float* __restrict__ wts, const float* __restrict__ alpha,
+    const ParView* __restrict__ views, mbir_par_geom g, TS* __restrict__ sino,
+    const TS* __restrict__ wts, const float* __restrict__ alpha,
     double* __restrict__ partials) {
   using L = FwdLanes<LPC, VR>;
   using Cfg = BinCfg<TWO>;
@@ -722,8 +756,10 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
       const int n = c0 + unit + k * L::UNITS;
       if (n >= C) continue;
       if (MODE == FWD_WRITE_VRC) {
+        if constexpr (sizeof(TS) == sizeof(float)) {
 #pragma unroll
-        for (int q = 0; q < VR; ++q) sino[((size_t)v * R + rr + q) * C + n] = acc[k][q];
+          for (int q = 0; q < VR; ++q) sino[((size_t)v * R + rr + q) * C + n] = acc[k][q];
+        }
       } else {
         fwd_epilogue_vcr<VR, MODE, WEIGHTED>(acc[k], ((size_t)v * C + n) * R + rr, sino, wts,
                                              al, lin, quad);
@@ -758,36 +794,44 @@ static inline int back_pix_per_wg(int RW) {
   return (kBlock / MBIR_WAVE) * (MBIR_WAVE / RW) * (RW >= 4 ? 8 : 4);
 }
 
+// Sinogram storage: float32, or (cp == 1 only) bfloat16 bits.
 template <int RW, int NT>
-static int launch_back_rw(const float* sino, const float* w, const int64_t* pix, int P,
+static int launch_back_rw(const void* sino, const void* w, const int64_t* pix, int P,
                           const ParView* views, const mbir_par_geom& g, int cp, int ns,
-                          float* out, hipStream_t s) {
+                          float* out, hipStream_t s, bool bf16) {
   const int pwg = back_pix_per_wg(RW);
   const int vps = (g.num_views + ns - 1) / ns;
   dim3 grid((P + pwg - 1) / pwg, ns, (g.num_slices + RW - 1) / RW);
-  if (cp == 1) {
-    if (w) par_back_kernel<RW, NT, 1, true><<<grid, kBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
-    else par_back_kernel<RW, NT, 1, false><<<grid, kBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
+  const float* sf = static_cast<const float*>(sino);
+  const float* wf = static_cast<const float*>(w);
+  const bf16_bits* sh = static_cast<const bf16_bits*>(sino);
+  const bf16_bits* wh = static_cast<const bf16_bits*>(w);
+  if (bf16) {
+    if (w) par_back_kernel<RW, NT, 1, true, bf16_bits><<<grid, kBlock, 0, s>>>(sh, wh, pix, P, views, g, vps, out);
+    else par_back_kernel<RW, NT, 1, false, bf16_bits><<<grid, kBlock, 0, s>>>(sh, wh, pix, P, views, g, vps, out);
+  } else if (cp == 1) {
+    if (w) par_back_kernel<RW, NT, 1, true, float><<<grid, kBlock, 0, s>>>(sf, wf, pix, P, views, g, vps, out);
+    else par_back_kernel<RW, NT, 1, false, float><<<grid, kBlock, 0, s>>>(sf, wf, pix, P, views, g, vps, out);
   } else {
-    if (w) par_back_kernel<RW, NT, 2, true><<<grid, kBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
-    else par_back_kernel<RW, NT, 2, false><<<grid, kBlock, 0, s>>>(sino, w, pix, P, views, g, vps, out);
+    if (w) par_back_kernel<RW, NT, 2, true, float><<<grid, kBlock, 0, s>>>(sf, wf, pix, P, views, g, vps, out);
+    else par_back_kernel<RW, NT, 2, false, float><<<grid, kBlock, 0, s>>>(sf, wf, pix, P, views, g, vps, out);
   }
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }
 
 template <int NT>
-static int launch_back_nt(const float* sino, const float* w, const int64_t* pix, int P,
+static int launch_back_nt(const void* sino, const void* w, const int64_t* pix, int P,
                           const ParView* views, const mbir_par_geom& g, int cp, int ns,
-                          float* out, hipStream_t s) {
+                          float* out, hipStream_t s, bool bf16) {
   switch (back_rw(g.num_slices)) {
-    case 64: return launch_back_rw<64, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    case 32: return launch_back_rw<32, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    case 16: return launch_back_rw<16, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    case 8: return launch_back_rw<8, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    case 4: return launch_back_rw<4, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    case 2: return launch_back_rw<2, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
-    default: return launch_back_rw<1, NT>(sino, w, pix, P, views, g, cp, ns, out, s);
+    case 64: return launch_back_rw<64, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    case 32: return launch_back_rw<32, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    case 16: return launch_back_rw<16, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    case 8: return launch_back_rw<8, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    case 4: return launch_back_rw<4, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    case 2: return launch_back_rw<2, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
+    default: return launch_back_rw<1, NT>(sino, w, pix, P, views, g, cp, ns, out, s, bf16);
   }
 }
 // Forward dispatch: the ray form when `runs` is given, the binned form
@@ -801,11 +845,12 @@ struct FwdArgs {
   const int* runs;
   const ParView* views;
   mbir_par_geom g;
-  float* sino;
-  const float* w;
+  void* sino;
+  const void* w;
   const float* alpha;
   double* partials;
   hipStream_t s;
+  bool bf16;  // sinogram / weights stored as bfloat16 (binned line search / update)
 };
 
 // Lane configurations (lanes per unit, rows per lane).  VR = 4 or 8 rows per
@@ -848,16 +893,39 @@ static void launch_fwd_kernel(const FwdArgs& a) {
     constexpr int CHW = (kBlock / LPC) * CPS;
     dim3 grid((g.num_channels + CHW - 1) / CHW, g.num_views, z);
     par_fwd_ray_kernel<LPC, VR, CPS, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
-        a.values, a.runs, a.views, g, a.sino, a.w, a.alpha, a.partials);
+        a.values, a.runs, a.views, g, static_cast<float*>(a.sino),
+        static_cast<const float*>(a.w), a.alpha, a.partials);
   } else {
     dim3 grid(g.num_views, (g.num_channels + kBlock - 1) / kBlock, z);
-    par_fwd_bin_kernel<LPC, VR, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
-        a.values, a.pix, a.P, a.views, g, a.sino, a.w, a.alpha, a.partials);
+    par_fwd_bin_kernel<LPC, VR, TWO, MODE, WEIGHTED, float><<<grid, kBlock, 0, a.s>>>(
+        a.values, a.pix, a.P, a.views, g, static_cast<float*>(a.sino),
+        static_cast<const float*>(a.w), a.alpha, a.partials);
   }
+}
+
+template <int LPC, int VR, bool TWO, int MODE, bool WEIGHTED>
+static void launch_fwd_bf16(const FwdArgs& a) {
+  const mbir_par_geom& g = a.g;
+  constexpr int ROWS = LPC * VR;
+  dim3 grid(g.num_views, (g.num_channels + kBlock - 1) / kBlock,
+            (g.num_slices + ROWS - 1) / ROWS);
+  par_fwd_bin_kernel<LPC, VR, TWO, MODE, WEIGHTED, bf16_bits><<<grid, kBlock, 0, a.s>>>(
+      a.values, a.pix, a.P, a.views, g, static_cast<bf16_bits*>(a.sino),
+      static_cast<const bf16_bits*>(a.w), a.alpha, a.partials);
 }
 
 template <int LPC, int VR, bool TWO>
 static int launch_fwd_mode(const FwdArgs& a) {
+  if (a.bf16) {
+    if (a.mode == FWD_LINESEARCH) {
+      if (a.w) launch_fwd_bf16<LPC, VR, TWO, FWD_LINESEARCH, true>(a);
+      else launch_fwd_bf16<LPC, VR, TWO, FWD_LINESEARCH, false>(a);
+    } else {
+      launch_fwd_bf16<LPC, VR, TWO, FWD_UPDATE, false>(a);
+    }
+    MBIR_CHECK_LAUNCH();
+    return MBIR_OK;
+  }
   switch (a.mode) {
     case FWD_WRITE_VCR: launch_fwd_kernel<LPC, VR, TWO, FWD_WRITE_VCR, false>(a); break;
     case FWD_WRITE_VRC: launch_fwd_kernel<LPC, VR, TWO, FWD_WRITE_VRC, false>(a); break;
@@ -892,7 +960,10 @@ static inline bool aligned16(const void* p) {
 
 static int launch_fwd(const FwdArgs& a) {
   const bool vec = fwd_vec(a.g);
+  // bfloat16 rows are read 4 at a time (8 bytes); 16-byte alignment covers it.
   if (vec && !(aligned16(a.values) && aligned16(a.sino) && (!a.w || aligned16(a.w))))
+    return MBIR_ERR_ARG;
+  if (a.bf16 && (a.runs || a.mode == FWD_WRITE_VCR || a.mode == FWD_WRITE_VRC))
     return MBIR_ERR_ARG;
   return a.g.two_tap ? launch_fwd_lanes<true>(a, vec) : launch_fwd_lanes<false>(a, vec);
 }
@@ -918,9 +989,9 @@ int32_t mbir_par_back_splits(const mbir_par_geom* g, int32_t P) {
   return (int32_t)ns;
 }
 
-int mbir_par_back(const float* sino, const float* w, const int64_t* pix, int32_t P,
-                  const float* view_table, const mbir_par_geom* g, int32_t cp,
-                  int32_t ns, float* out, mbir_stream_t stream) {
+static int par_back_any(const void* sino, const void* w, const int64_t* pix, int32_t P,
+                        const float* view_table, const mbir_par_geom* g, int32_t cp, int32_t ns,
+                        float* out, mbir_stream_t stream, bool bf16) {
   int st = check_geom(g);
   if (st) return st;
   if (!sino || !pix || !view_table || !out || P < 0) return MBIR_ERR_ARG;
@@ -929,14 +1000,26 @@ int mbir_par_back(const float* sino, const float* w, const int64_t* pix, int32_t
   if (P == 0) return MBIR_OK;
   hipStream_t s = (hipStream_t)stream;
   const ParView* views = reinterpret_cast<const ParView*>(view_table);
-  if (g->two_tap) return launch_back_nt<2>(sino, w, pix, P, views, *g, cp, ns, out, s);
+  if (g->two_tap) return launch_back_nt<2>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
   switch (2 * g->psf_radius + 1) {
-    case 1: return launch_back_nt<1>(sino, w, pix, P, views, *g, cp, ns, out, s);
-    case 3: return launch_back_nt<3>(sino, w, pix, P, views, *g, cp, ns, out, s);
-    case 5: return launch_back_nt<5>(sino, w, pix, P, views, *g, cp, ns, out, s);
-    case 7: return launch_back_nt<7>(sino, w, pix, P, views, *g, cp, ns, out, s);
-    default: return launch_back_nt<9>(sino, w, pix, P, views, *g, cp, ns, out, s);
+    case 1: return launch_back_nt<1>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
+    case 3: return launch_back_nt<3>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
+    case 5: return launch_back_nt<5>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
+    case 7: return launch_back_nt<7>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
+    default: return launch_back_nt<9>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
   }
+}
+
+int mbir_par_back(const float* sino, const float* w, const int64_t* pix, int32_t P,
+                  const float* view_table, const mbir_par_geom* g, int32_t cp,
+                  int32_t ns, float* out, mbir_stream_t stream) {
+  return par_back_any(sino, w, pix, P, view_table, g, cp, ns, out, stream, false);
+}
+
+int mbir_par_back_bf16(const uint16_t* sino, const uint16_t* w, const int64_t* pix, int32_t P,
+                       const float* view_table, const mbir_par_geom* g, int32_t ns,
+                       float* out, mbir_stream_t stream) {
+  return par_back_any(sino, w, pix, P, view_table, g, 1, ns, out, stream, true);
 }
 
 int mbir_reduce_splits(const float* parts, int32_t ns, int64_t n, float* out,
@@ -959,7 +1042,7 @@ int mbir_par_forward(const float* values, const int64_t* pix, int32_t P, const i
   if (layout != MBIR_LAYOUT_VRC && layout != MBIR_LAYOUT_VCR) return MBIR_ERR_ARG;
   FwdArgs a{layout == MBIR_LAYOUT_VRC ? FWD_WRITE_VRC : FWD_WRITE_VCR,
             values, pix, P, runs, reinterpret_cast<const ParView*>(view_table), *g, out,
-            nullptr, nullptr, nullptr, (hipStream_t)stream};
+            nullptr, nullptr, nullptr, (hipStream_t)stream, false};
   return launch_fwd(a);
 }
 
@@ -978,7 +1061,7 @@ int mbir_par_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
     return MBIR_ERR_ARG;
   FwdArgs a{FWD_LINESEARCH, delta, pix, P, runs,
             reinterpret_cast<const ParView*>(view_table), *g, const_cast<float*>(error_vcr), w,
-            nullptr, partials, (hipStream_t)stream};
+            nullptr, partials, (hipStream_t)stream, false};
   return launch_fwd(a);
 }
 
@@ -990,7 +1073,31 @@ int mbir_par_fwd_update(const float* delta, const int64_t* pix, int32_t P, const
   if (!delta || !view_table || !error_vcr || !alpha || P < 0 || (!pix && !runs))
     return MBIR_ERR_ARG;
   FwdArgs a{FWD_UPDATE, delta, pix, P, runs, reinterpret_cast<const ParView*>(view_table), *g,
-            error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream};
+            error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream, false};
+  return launch_fwd(a);
+}
+
+int mbir_par_fwd_linesearch_bf16(const float* delta, const int64_t* pix, int32_t P,
+                                 const float* view_table, const mbir_par_geom* g,
+                                 const uint16_t* error_vcr, const uint16_t* w, double* partials,
+                                 mbir_stream_t stream) {
+  int st = check_geom(g);
+  if (st) return st;
+  if (!delta || !pix || !view_table || !error_vcr || !partials || P < 0) return MBIR_ERR_ARG;
+  FwdArgs a{FWD_LINESEARCH, delta, pix, P, nullptr,
+            reinterpret_cast<const ParView*>(view_table), *g, const_cast<uint16_t*>(error_vcr),
+            w, nullptr, partials, (hipStream_t)stream, true};
+  return launch_fwd(a);
+}
+
+int mbir_par_fwd_update_bf16(const float* delta, const int64_t* pix, int32_t P,
+                             const float* view_table, const mbir_par_geom* g, const float* alpha,
+                             uint16_t* error_vcr, mbir_stream_t stream) {
+  int st = check_geom(g);
+  if (st) return st;
+  if (!delta || !pix || !view_table || !error_vcr || !alpha || P < 0) return MBIR_ERR_ARG;
+  FwdArgs a{FWD_UPDATE, delta, pix, P, nullptr, reinterpret_cast<const ParView*>(view_table),
+            *g, error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream, true};
   return launch_fwd(a);
 }
 

@@ -239,20 +239,30 @@ static inline int apply_blocks(int P) {
   return nb > 1024 ? 1024 : (nb < 1 ? 1 : nb);
 }
 
+// Storage conversion (float32 <-> bfloat16, round to nearest even).
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kThreads) void convert_kernel(const TI* __restrict__ in,
+                                                           TO* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+    st_s(out + i, ld_s(in + i));
+}
+
 // ---- reductions ------------------------------------------------------------
 constexpr int kRedItems = 8;  // elements per thread per block pass
 
-__global__ __launch_bounds__(kThreads) void sumsq_partial_kernel(const float* __restrict__ a,
-                                                                 const float* __restrict__ w,
+template <typename TS>
+__global__ __launch_bounds__(kThreads) void sumsq_partial_kernel(const TS* __restrict__ a,
+                                                                 const TS* __restrict__ w,
                                                                  int64_t n,
                                                                  double* __restrict__ part) {
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kThreads) {
-    const float v = a[i];
+    const float v = ld_s(a + i);
     const float vv = v * v;
     if (w) {
-      const float wv = w[i];
+      const float wv = ld_s(w + i);
       s0 += (double)(vv * wv);
       s2 += (double)wv;
     } else {
@@ -416,9 +426,41 @@ int mbir_sumsq_f64(const float* a, const float* w, int64_t n, double* scratch, d
   if (!a || !scratch || !out || n < 0) return MBIR_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   const int nb = red_blocks(n);
-  sumsq_partial_kernel<<<nb, kThreads, 0, s>>>(a, w, n, scratch);
+  sumsq_partial_kernel<float><<<nb, kThreads, 0, s>>>(a, w, n, scratch);
   MBIR_CHECK_LAUNCH();
   final3_kernel<<<1, kThreads, 0, s>>>(scratch, nb, 3, out);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int mbir_sumsq_bf16(const uint16_t* a, const uint16_t* w, int64_t n, double* scratch,
+                    double* out, mbir_stream_t stream) {
+  if (!a || !scratch || !out || n < 0) return MBIR_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = red_blocks(n);
+  sumsq_partial_kernel<bf16_bits><<<nb, kThreads, 0, s>>>(a, w, n, scratch);
+  MBIR_CHECK_LAUNCH();
+  final3_kernel<<<1, kThreads, 0, s>>>(scratch, nb, 3, out);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int mbir_f32_to_bf16(const float* in, uint16_t* out, int64_t n, mbir_stream_t stream) {
+  if (!in || !out || n < 0) return MBIR_ERR_ARG;
+  if (n == 0) return MBIR_OK;
+  int nb = (int)((n + kThreads - 1) / kThreads);
+  if (nb > 8192) nb = 8192;
+  convert_kernel<float, bf16_bits><<<nb, kThreads, 0, (hipStream_t)stream>>>(in, out, n);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int mbir_bf16_to_f32(const uint16_t* in, float* out, int64_t n, mbir_stream_t stream) {
+  if (!in || !out || n < 0) return MBIR_ERR_ARG;
+  if (n == 0) return MBIR_OK;
+  int nb = (int)((n + kThreads - 1) / kThreads);
+  if (nb > 8192) nb = 8192;
+  convert_kernel<bf16_bits, float><<<nb, kThreads, 0, (hipStream_t)stream>>>(in, out, n);
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }

@@ -26,6 +26,7 @@ int check_job(const mbir_vcd_job* j) {
       !j->alpha || !j->apply_scratch || !j->red_scratch)
     return MBIR_ERR_ARG;
   if (j->dc_profile && !j->dc_scratch) return MBIR_ERR_ARG;
+  if (j->sino_dtype != MBIR_DTYPE_F32 && j->sino_dtype != MBIR_DTYPE_BF16) return MBIR_ERR_ARG;
   if (j->prm.num_slices <= 0 || j->num_subsets <= 0) return MBIR_ERR_SHAPE;
   return MBIR_OK;
 }
@@ -42,12 +43,24 @@ int subset_step(const mbir_vcd_job* j, int k, double* stats, hipStream_t s) {
   const int64_t* pix = j->subset_pixels[k];
   const int32_t* runs = j->subset_runs ? j->subset_runs[k] : nullptr;
   const bool par = j->geometry == MBIR_GEOMETRY_PARALLEL;
+  const bool h = j->sino_dtype == MBIR_DTYPE_BF16;
+  // bfloat16 storage: the error sinogram and weights are 16-bit words.
+  const uint16_t* e16 = reinterpret_cast<const uint16_t*>(j->error_vcr);
+  uint16_t* e16w = reinterpret_cast<uint16_t*>(j->error_vcr);
+  const uint16_t* w16 = reinterpret_cast<const uint16_t*>(j->weights_vcr);
   if (P <= 0) return MBIR_OK;
+  if (h) runs = nullptr;  // the binned forward form serves bfloat16 storage
   const int ns = par ? mbir_par_back_splits(j->par_geom, P) : mbir_cone_back_splits(j->cone_geom, P);
   if ((int64_t)ns * P * S > j->back_parts_capacity) return MBIR_ERR_WORKSPACE;
-  if (par)
+  if (par && h)
+    MBIR_TRY(mbir_par_back_bf16(e16, w16, pix, P, j->view_table, j->par_geom, ns, j->back_parts,
+                                s));
+  else if (par)
     MBIR_TRY(mbir_par_back(j->error_vcr, j->weights_vcr, pix, P, j->view_table, j->par_geom, 1,
                            ns, j->back_parts, s));
+  else if (h)
+    MBIR_TRY(mbir_cone_back_bf16(e16, w16, pix, P, j->view_table, j->cone_geom, ns,
+                                 j->back_parts, s));
   else
     MBIR_TRY(mbir_cone_back(j->error_vcr, j->weights_vcr, pix, P, j->view_table, j->cone_geom,
                             1, ns, j->back_parts, s));
@@ -61,20 +74,34 @@ int subset_step(const mbir_vcd_job* j, int k, double* stats, hipStream_t s) {
                                 &j->prm, j->delta, j->dir_parts, s));
   int n_fwd;
   if (par) {
-    MBIR_TRY(mbir_par_fwd_linesearch(j->delta, pix, P, runs, j->view_table, j->par_geom,
-                                     j->error_vcr, j->weights_vcr, j->fwd_parts, s));
+    if (h)
+      MBIR_TRY(mbir_par_fwd_linesearch_bf16(j->delta, pix, P, j->view_table, j->par_geom, e16,
+                                            w16, j->fwd_parts, s));
+    else
+      MBIR_TRY(mbir_par_fwd_linesearch(j->delta, pix, P, runs, j->view_table, j->par_geom,
+                                       j->error_vcr, j->weights_vcr, j->fwd_parts, s));
     n_fwd = mbir_par_fwd_partials_count(j->par_geom, runs != nullptr);
   } else {
-    MBIR_TRY(mbir_cone_fwd_linesearch(j->delta, pix, P, j->view_table, j->cone_geom,
-                                      j->error_vcr, j->weights_vcr, j->fwd_parts, s));
+    if (h)
+      MBIR_TRY(mbir_cone_fwd_linesearch_bf16(j->delta, pix, P, j->view_table, j->cone_geom, e16,
+                                             w16, j->fwd_parts, s));
+    else
+      MBIR_TRY(mbir_cone_fwd_linesearch(j->delta, pix, P, j->view_table, j->cone_geom,
+                                        j->error_vcr, j->weights_vcr, j->fwd_parts, s));
     n_fwd = mbir_cone_fwd_partials_count(j->cone_geom);
   }
   MBIR_TRY(mbir_vcd_alpha(j->dir_parts, n_dir, j->fwd_parts, n_fwd, &j->prm, j->alpha,
                           stats + S + 1, s));
   if (j->positivity) MBIR_TRY(mbir_vcd_positivity(j->flat_recon, pix, P, S, j->alpha, j->delta, s));
-  if (par)
+  if (par && h)
+    MBIR_TRY(mbir_par_fwd_update_bf16(j->delta, pix, P, j->view_table, j->par_geom, j->alpha,
+                                      e16w, s));
+  else if (par)
     MBIR_TRY(mbir_par_fwd_update(j->delta, pix, P, runs, j->view_table, j->par_geom, j->alpha,
                                  j->error_vcr, s));
+  else if (h)
+    MBIR_TRY(mbir_cone_fwd_update_bf16(j->delta, pix, P, j->view_table, j->cone_geom, j->alpha,
+                                       e16w, s));
   else
     MBIR_TRY(mbir_cone_fwd_update(j->delta, pix, P, j->view_table, j->cone_geom, j->alpha,
                                   j->error_vcr, s));
@@ -97,8 +124,13 @@ int mbir_vcd_run(const mbir_vcd_job* job, const int32_t* subsets, int32_t num_st
   }
   if (end_iteration) {
     const int S = job->prm.num_slices;
-    MBIR_TRY(mbir_sumsq_f64(job->error_vcr, job->weights_vcr, job->sino_size, job->red_scratch,
-                            stats + S + 2, s));
+    if (job->sino_dtype == MBIR_DTYPE_BF16)
+      MBIR_TRY(mbir_sumsq_bf16(reinterpret_cast<const uint16_t*>(job->error_vcr),
+                               reinterpret_cast<const uint16_t*>(job->weights_vcr),
+                               job->sino_size, job->red_scratch, stats + S + 2, s));
+    else
+      MBIR_TRY(mbir_sumsq_f64(job->error_vcr, job->weights_vcr, job->sino_size,
+                              job->red_scratch, stats + S + 2, s));
     MBIR_TRY(mbir_sumabs_f64(job->flat_recon, job->recon_size, job->red_scratch, stats + S + 5,
                              s));
   }

@@ -495,7 +495,7 @@ class TomographyModel(ParameterHandler):
         losses = (fm_rmse[:num_iters], np.zeros(num_iters), nmae[:num_iters],
                   alpha_values[:num_iters], dnorm[:num_iters])
         if return_checkpoint:
-            ckpt = {'error_sinogram': self._vcr_to_vrc(engine.e), 'fm_hessian':
+            ckpt = {'error_sinogram': self._vcr_to_vrc(_to_f32(engine.e)), 'fm_hessian':
                     hess.reshape(recon_shape[0], recon_shape[1], S)}
             return recon3d, losses, ckpt
         return recon3d, losses
@@ -560,6 +560,13 @@ class TomographyModel(ParameterHandler):
             hess = self._recon_to_device(fm_hessian).reshape(n_grid, S).contiguous()
 
         prm = self._vcd_params(prox=prox_input is not None)
+        storage = self.get_params('sinogram_storage')
+        if storage not in ('float32', 'bfloat16'):
+            raise ValueError(f"sinogram_storage must be 'float32' or 'bfloat16'; got {storage!r}")
+        if storage == 'bfloat16':
+            e_vcr = _to_bf16(e_vcr)
+            if w_vcr is not None:
+                w_vcr = _to_bf16(w_vcr)
         engine = _VcdEngine(self, ops, prm, x_flat, e_vcr, w_vcr, hess, prox_flat, partitions)
         return engine, hess
 
@@ -689,6 +696,23 @@ def generate_direct_recon_filter(num_channels, filter_name='ramp'):
     return ((1 / 2) * np.sinc(n) - (1 / 4) * (np.sinc(n / 2)) ** 2).astype(np.float32)
 
 
+def _to_bf16(t):
+    """float32 device tensor -> bfloat16 (round to nearest even) via the library."""
+    out = torch.empty(t.shape, dtype=torch.bfloat16, device=t.device)
+    _lib.check(_lib.lib().mbir_f32_to_bf16(_lib.ptr(t.contiguous()), _lib.ptr(out), t.numel(),
+                                           _lib.stream_ptr()), 'to bf16')
+    return out
+
+
+def _to_f32(t):
+    if t.dtype == torch.float32:
+        return t
+    out = torch.empty(t.shape, dtype=torch.float32, device=t.device)
+    _lib.check(_lib.lib().mbir_bf16_to_f32(_lib.ptr(t.contiguous()), _lib.ptr(out), t.numel(),
+                                           _lib.stream_ptr()), 'to f32')
+    return out
+
+
 class _VcdEngine:
     """Device-resident VCD state handed to the native schedule runner
     (mbir_vcd_run): one library call per iteration issues every subset step
@@ -755,6 +779,8 @@ class _VcdEngine:
         job.geometry = ops.geometry_id
         job.positivity = int(bool(model.get_params('positivity_flag')))
         job.num_subsets = n_sub
+        job.sino_dtype = (_lib.MBIR_DTYPE_BF16 if e.dtype == torch.bfloat16
+                          else _lib.MBIR_DTYPE_F32)
         if ops.geometry_id == _lib.MBIR_GEOMETRY_PARALLEL:
             job.par_geom = ctypes.pointer(ops.geom)
         else:
