@@ -83,4 +83,26 @@ __device__ __forceinline__ void block_sum2_f64(double& a, double& b) {
   }
 }
 
+// Rank of this lane's bin b (b < 0: no entry) among the lanes below it with
+// the same bin, the size of its group, and whether it is the group's lowest
+// lane.  One ballot per distinct bin in the wavefront.
+__device__ __forceinline__ void wave_bin_rank(int b, int& rank, int& gsize, bool& leader) {
+  const int lane = threadIdx.x & (MBIR_WAVE - 1);
+  unsigned long long todo = __ballot(b >= 0);
+  rank = 0;
+  gsize = 0;
+  leader = false;
+  while (todo) {
+    const int l = __ffsll(todo) - 1;
+    const int bl = __shfl(b, l, MBIR_WAVE);
+    const unsigned long long m = __ballot(b == bl);
+    if (b == bl) {
+      rank = __popcll(m & ((1ull << lane) - 1ull));
+      gsize = __popcll(m);
+      leader = lane == l;
+    }
+    todo &= ~m;
+  }
+}
+
 }  // namespace mbir

@@ -16,11 +16,12 @@
 // taps of consecutive slices read consecutive detector rows of the
 // channel-major sinogram e[v, n, m].
 //
-// Forward (cone_beam.py:161-234): a workgroup owns one view and a band of RC
-// detector rows, with the (C x RC) tile in LDS; lanes map to rows, each row
-// gathers its voxel taps (detector-side vertical fan) and adds the
-// horizontal-fan taps into the tile with LDS float atomics.  Epilogues as in
-// the parallel projector: write, FP64 line search, or error update.
+// Forward (cone_beam.py:161-234): an atomic-free binned gather -- a
+// workgroup owns one view, 64 channels and a band of detector rows (one lane
+// per row); the pixel list is counting-sorted by channel in LDS and each
+// row gathers its voxel taps (detector-side vertical fan) per channel entry.
+// Epilogues as in the parallel projector: write, FP64 line search, or error
+// update.
 #include "common.h"
 
 namespace mbir {
@@ -198,109 +199,225 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Forward projection into a per-view row-band tile.
+// Forward projection: binned gather (atomic-free, deterministic).
+// Workgroup = (view, kConeCB channels, row chunk of LPC detector rows); lane
+// rl of a unit owns detector row m = r0 + rl.  Chunks of the pixel list are
+// counting-sorted by channel in LDS with a placement fixed by the list
+// (short bins: scatter then insertion sort by position; long bins: wave
+// ballot ranks), the pixel's vertical map (m0, Wr, magnification) kept per
+// list position.  Unit u gathers the bins of channels c0 + u + k*UNITS in
+// rounds: for each entry the lane forms the reference's detector-side
+// vertical fan sum for its row -- slices round(k_m) +- bp_psf with
+// k_m = (m - m0)/Wr, trapezoid weights, values scaled by 1/cos(phi) -- and
+// adds the entry's horizontal weight times it (cone_beam.py:161-234).
+// Epilogues as in the parallel projector: write, FP64 line search, or error
+// update.
 enum ConeFwdMode { CF_WRITE_VCR = 0, CF_WRITE_VRC = 1, CF_LINESEARCH = 2, CF_UPDATE = 3 };
 
-template <int RC, int NH, int NK, int MODE, bool WEIGHTED, typename TS>
-__global__ __launch_bounds__(kCBlock) void cone_fwd_tile_kernel(
+constexpr int kConeCB = 64;     // channels per workgroup
+constexpr int kConeShortBin = 24;
+
+template <int NH>
+struct ConeBinCfg {
+  static constexpr int CH = NH <= 3 ? 256 : 128;  // pixels per chunk
+  static constexpr int ENT = CH * NH;             // entry capacity
+};
+
+template <int LPC, int NH, int NK, int MODE, bool WEIGHTED, typename TS>
+__global__ __launch_bounds__(kCBlock) void cone_fwd_bin_kernel(
     const float* __restrict__ values, const int64_t* __restrict__ pix, int n_pix,
     const ConeView* __restrict__ views, mbir_cone_geom g, TS* __restrict__ sino,
     const TS* __restrict__ wts, const float* __restrict__ alpha,
     double* __restrict__ partials) {
-  constexpr int TSTR = RC + 1;
-  constexpr int SLOTS = MBIR_WAVE / RC;
-  constexpr int REC = NH + 4;
-  extern __shared__ float smem[];
-  const int C = g.num_channels, R = g.num_det_rows, S = g.num_slices;
-  float* tile = smem;                      // C x TSTR
-  float* s_rec = tile + (size_t)C * TSTR;  // REC x kCBlock
+  using Cfg = ConeBinCfg<NH>;
+  constexpr int SPW = MBIR_WAVE / LPC;
+  constexpr int UNITS = kCBlock / LPC;
+  constexpr int CPU = kConeCB / UNITS;  // channels per unit (LPC >= 4)
+  constexpr int NW = kCBlock / MBIR_WAVE;
+  __shared__ int s_cnt[kConeCB + 1];
+  __shared__ int s_off[kConeCB + 1];
+  __shared__ int s_wcnt[NW][kConeCB];
+  __shared__ int s_ej[Cfg::ENT];
+  __shared__ float s_ea[Cfg::ENT];
+  __shared__ float s_m0[Cfg::CH], s_wr[Cfg::CH], s_mag[Cfg::CH];
 
-  const int v = blockIdx.x, m_lo = blockIdx.y * RC;
   const int tid = threadIdx.x;
   const int lane = tid & (MBIR_WAVE - 1), wave = tid / MBIR_WAVE;
-  const int slot = lane / RC, rl = lane % RC;
-  const int m = m_lo + rl;
+  const int unit = wave * SPW + lane / LPC, rl = lane % LPC;
+  const int v = blockIdx.x, c0 = blockIdx.y * kConeCB, r0 = blockIdx.z * LPC;
+  const int S = g.num_slices, C = g.num_channels, R = g.num_det_rows;
+  const int m = r0 + rl;
+  const bool row_ok = m < R;
   const float mf = (float)m;
   const ConeView vw = views[v];
   const float sdd = g.source_detector_dist;
+  if (tid < kConeCB) {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s_wcnt[w][tid] = 0;
+  }
 
-  for (int i = tid; i < C * TSTR; i += kCBlock) tile[i] = 0.0f;
+  float acc[CPU];
+#pragma unroll
+  for (int k = 0; k < CPU; ++k) acc[k] = 0.0f;
 
-  int64_t p_next = tid < n_pix ? pix[tid] : 0;
-  for (int pc = 0; pc < n_pix; pc += kCBlock) {
-    int base = 0;
+  for (int pc = 0; pc < n_pix; pc += Cfg::CH) {
+    // Geometry of pixel pc + tid; its horizontal taps landing in the block count.
+    int bin[NH];
     float a[NH];
-    ConePixel q = {0, 1, 0, 1, -1e9f, 1};
-    const bool valid = pc + tid < n_pix;
-    if (valid) {
-      cone_pixel(p_next, vw, g, q);
-      cone_htaps<NH>(q, g, base, a);
+    {
+      int base = 0;
+      ConePixel q = {0, 1, 0, 1, -1e9f, 1};
+      const bool valid = tid < Cfg::CH && pc + tid < n_pix;
+      if (valid) {
+        cone_pixel(pix[pc + tid], vw, g, q);
+        cone_htaps<NH>(q, g, base, a);
+      } else {
+#pragma unroll
+        for (int t = 0; t < NH; ++t) a[t] = 0.0f;
+      }
+      if (tid < Cfg::CH) {
+        s_m0[tid] = q.m0;
+        s_wr[tid] = q.Wr;
+        s_mag[tid] = q.mag;
+      }
+      if (tid < kConeCB) s_cnt[tid] = 0;
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < NH; ++t) {
+        const int b = base + t - c0;
+        const bool in = a[t] != 0.0f && b >= 0 && b < kConeCB;
+        bin[t] = in ? b : -1;
+        if (in) atomicAdd(&s_cnt[b], 1);
+      }
+    }
+    __syncthreads();
+    int mine = 0;
+    if (wave == 0) {  // exclusive scan of the kConeCB == 64 bin counts
+      mine = s_cnt[lane];
+      int x = mine;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+      }
+      s_off[lane] = x - mine;
+      if (lane == 63) s_off[kConeCB] = x;
+    }
+    __syncthreads();
+    if (tid < kConeCB) s_cnt[tid] = s_off[tid];  // placement cursor
+    if (!__syncthreads_or(tid < kConeCB && mine > kConeShortBin)) {
+#pragma unroll
+      for (int t = 0; t < NH; ++t) {
+        const int b = bin[t];
+        if (b >= 0) {
+          const int slot = atomicAdd(&s_cnt[b], 1);
+          s_ej[slot] = tid;
+          s_ea[slot] = a[t];
+        }
+      }
+      __syncthreads();
+      if (tid < kConeCB) {
+        const int e0 = s_off[tid], e1 = s_off[tid + 1];
+        for (int e = e0 + 1; e < e1; ++e) {
+          const int kj = s_ej[e];
+          const float ka = s_ea[e];
+          int f = e - 1;
+          while (f >= e0 && s_ej[f] > kj) {
+            s_ej[f + 1] = s_ej[f];
+            s_ea[f + 1] = s_ea[f];
+            --f;
+          }
+          s_ej[f + 1] = kj;
+          s_ea[f + 1] = ka;
+        }
+      }
     } else {
 #pragma unroll
-      for (int t = 0; t < NH; ++t) a[t] = 0.0f;
-    }
-    __syncthreads();
-    s_rec[tid] = __int_as_float(base);
+      for (int t = 0; t < NH; ++t) {
+        const int b = bin[t];
+        int rank, gsize;
+        bool leader;
+        wave_bin_rank(b, rank, gsize, leader);
+        if (leader) s_wcnt[wave][b] = gsize;
+        if (!__syncthreads_or(b >= 0)) continue;
+        if (b >= 0) {
+          int slot = s_cnt[b] + rank;
+          for (int w = 0; w < wave; ++w) slot += s_wcnt[w][b];
+          s_ej[slot] = tid;
+          s_ea[slot] = a[t];
+        }
+        __syncthreads();
+        if (tid < kConeCB) {
+          int add = 0;
 #pragma unroll
-    for (int t = 0; t < NH; ++t) s_rec[(1 + t) * kCBlock + tid] = a[t];
-    s_rec[(1 + NH) * kCBlock + tid] = q.m0;
-    s_rec[(2 + NH) * kCBlock + tid] = q.Wr;
-    s_rec[(3 + NH) * kCBlock + tid] = q.mag;
-    __syncthreads();
-    if (pc + kCBlock + tid < n_pix) p_next = pix[pc + kCBlock + tid];
-    const int cnt = min(kCBlock, n_pix - pc);
-#pragma unroll 2
-    for (int jj = 0; jj < MBIR_WAVE; jj += SLOTS) {
-      const int j = wave * MBIR_WAVE + jj + slot;
-      const int jc = min(j, cnt - 1);
-      const float m0 = s_rec[(1 + NH) * kCBlock + j];
-      const float Wr = s_rec[(2 + NH) * kCBlock + j];
-      const float mag = s_rec[(3 + NH) * kCBlock + j];
-      // Detector row -> fractional slice (inverse of the affine map).
-      const float k_m = (mf - m0) / Wr;
-      const float kc = rintf(k_m);
-      const float m_p = Wr * (kc - k_m);
-      const float half = (Wr + 1.0f) / 2.0f, lmax = fminf(Wr, 1.0f);
-      const float* xp = values + (size_t)(pc + jc) * S;
-      float det = 0.0f;
-#pragma unroll
-      for (int tk = 0; tk < NK; ++tk) {
-        const int off = tk - (NK - 1) / 2;
-        const int ki = (int)kc + off;
-        float A = fminf(fmaxf(half - fabsf(m_p + Wr * (float)off), 0.0f), lmax);
-        const bool kv = ki >= 0 && ki < S && j < cnt && m < R;
-        A = kv ? A : 0.0f;
-        const int kcl = min(max(ki, 0), S - 1);
-        const float z = g.delta_voxel_slice * ((float)kcl - g.slice_center) + vw.z_off;
-        const float gv = xp[kcl] * inv_cos_phi(mag, z, sdd);
-        det = det + A * gv;
-      }
-      const int b = __float_as_int(s_rec[j]);
-#pragma unroll
-      for (int th = 0; th < NH; ++th) {
-        const int n = min(max(b + th, 0), C - 1);
-        atomicAdd(&tile[n * TSTR + rl], s_rec[(1 + th) * kCBlock + j] * det);
+          for (int w = 0; w < NW; ++w) {
+            add += s_wcnt[w][tid];
+            s_wcnt[w][tid] = 0;
+          }
+          s_cnt[tid] += add;
+        }
+        __syncthreads();
       }
     }
+    __syncthreads();
+    if (row_ok) {
+      int span[CPU];  // (bin start << 16) | bin length; ENT < 65536
+      int rounds = 0;
+#pragma unroll
+      for (int k = 0; k < CPU; ++k) {
+        const int ch = unit + k * UNITS;
+        const int e0 = s_off[ch], len = s_off[ch + 1] - e0;
+        span[k] = (e0 << 16) | len;
+        rounds = max(rounds, len);
+      }
+      for (int r = 0; r < rounds; ++r) {
+#pragma unroll
+        for (int k = 0; k < CPU; ++k) {
+          if (r < (span[k] & 0xffff)) {
+            const int e = (span[k] >> 16) + r;
+            const int j = s_ej[e];
+            const float m0 = s_m0[j], Wr = s_wr[j], mag = s_mag[j];
+            const float* xp = values + (size_t)(pc + j) * S;
+            // Detector row -> fractional slice (inverse of the affine map).
+            const float k_m = (mf - m0) / Wr;
+            const float kc = rintf(k_m);
+            const float m_p = Wr * (kc - k_m);
+            const float half = (Wr + 1.0f) / 2.0f, lmax = fminf(Wr, 1.0f);
+            float det = 0.0f;
+#pragma unroll
+            for (int tk = 0; tk < NK; ++tk) {
+              const int off = tk - (NK - 1) / 2;
+              const int ki = (int)kc + off;
+              float A = fminf(fmaxf(half - fabsf(m_p + Wr * (float)off), 0.0f), lmax);
+              A = (ki >= 0 && ki < S) ? A : 0.0f;
+              const int kcl = min(max(ki, 0), S - 1);
+              const float z = g.delta_voxel_slice * ((float)kcl - g.slice_center) + vw.z_off;
+              const float gv = xp[kcl] * inv_cos_phi(mag, z, sdd);
+              det = det + A * gv;
+            }
+            acc[k] = acc[k] + s_ea[e] * det;
+          }
+        }
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
 
-  if (MODE == CF_WRITE_VRC) {
-    for (int i = tid; i < C * RC; i += kCBlock) {
-      const int rr = i / C, c = i - rr * C;
-      if (m_lo + rr < R) st_s(sino + ((size_t)v * R + m_lo + rr) * C + c, tile[c * TSTR + rr]);
-    }
-  } else {
-    double lin = 0.0, quad = 0.0;
-    float al = 0.0f;
-    if (MODE == CF_UPDATE) al = alpha[0];
-    for (int i = tid; i < C * RC; i += kCBlock) {
-      const int c = i / RC, rr = i - c * RC;
-      if (m_lo + rr >= R) continue;
-      const size_t o = ((size_t)v * C + c) * R + m_lo + rr;
-      const float t = tile[c * TSTR + rr];
+  double lin = 0.0, quad = 0.0;
+  const float al = MODE == CF_UPDATE ? alpha[0] : 0.0f;
+  if (row_ok) {
+#pragma unroll
+    for (int k = 0; k < CPU; ++k) {
+      const int n = c0 + unit + k * UNITS;
+      if (n >= C) continue;
+      const float t = acc[k];
+      if (MODE == CF_WRITE_VRC) {
+        if constexpr (sizeof(TS) == sizeof(float)) sino[((size_t)v * R + m) * C + n] = t;
+        continue;
+      }
+      const size_t o = ((size_t)v * C + n) * R + m;
       if (MODE == CF_WRITE_VCR) {
-        st_s(sino + o, t);
+        if constexpr (sizeof(TS) == sizeof(float)) sino[o] = t;
       } else if (MODE == CF_LINESEARCH) {
         const float e = ld_s(sino + o);
         if (WEIGHTED) {
@@ -315,13 +432,13 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_tile_kernel(
         st_s(sino + o, ld_s(sino + o) - al * t);
       }
     }
-    if (MODE == CF_LINESEARCH) {
-      block_sum2_f64(lin, quad);
-      if (tid == 0) {
-        const size_t bi = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-        partials[2 * bi] = lin;
-        partials[2 * bi + 1] = quad;
-      }
+  }
+  if (MODE == CF_LINESEARCH) {
+    block_sum2_f64(lin, quad);
+    if (tid == 0) {
+      const size_t bi = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+      partials[2 * bi] = lin;
+      partials[2 * bi + 1] = quad;
     }
   }
 }
@@ -348,15 +465,11 @@ static inline int cone_pix_per_wg(int RW) {
   return (kCBlock / MBIR_WAVE) * (MBIR_WAVE / RW) * (RW >= 4 ? 4 : 2);
 }
 
-static inline size_t cone_fwd_lds(int C, int rc, int nh) {
-  return ((size_t)C * (rc + 1) + (size_t)kCBlock * (nh + 4)) * 4;
-}
-
-static inline int cone_fwd_rc(const mbir_cone_geom* g) {
-  int rc = g->num_det_rows >= 64 ? 64 : cnext_pow2(g->num_det_rows);
-  const int nh = 2 * g->psf_radius + 1;
-  while (rc > 1 && cone_fwd_lds(g->num_channels, rc, nh) > 60 * 1024) rc >>= 1;
-  return rc;
+// Detector rows per forward workgroup (one lane each): the next power of
+// two >= R, between 4 and 64.
+static inline int cone_fwd_lpc(int R) {
+  const int p = cnext_pow2(R);
+  return p < 4 ? 4 : (p > 64 ? 64 : p);
 }
 
 // Sinogram storage: float32, or bfloat16 bits (back with cp == 1, forward
@@ -418,33 +531,33 @@ struct ConeFwdArgs {
   bool bf16;
 };
 
-template <int RC, int NH, int NK, int MODE, bool WEIGHTED, typename TS>
+template <int LPC, int NH, int NK, int MODE, bool WEIGHTED, typename TS>
 static void cone_fwd_kernel_launch(const ConeFwdArgs& a) {
-  dim3 grid(a.g.num_views, (a.g.num_det_rows + RC - 1) / RC);
-  const size_t lds = cone_fwd_lds(a.g.num_channels, RC, NH);
-  cone_fwd_tile_kernel<RC, NH, NK, MODE, WEIGHTED, TS><<<grid, kCBlock, lds, a.s>>>(
+  dim3 grid(a.g.num_views, (a.g.num_channels + kConeCB - 1) / kConeCB,
+            (a.g.num_det_rows + LPC - 1) / LPC);
+  cone_fwd_bin_kernel<LPC, NH, NK, MODE, WEIGHTED, TS><<<grid, kCBlock, 0, a.s>>>(
       a.values, a.pix, a.P, a.views, a.g, static_cast<TS*>(a.sino),
       static_cast<const TS*>(a.w), a.alpha, a.partials);
 }
 
-template <int RC, int NH, int NK>
+template <int LPC, int NH, int NK>
 static int cone_fwd_launch3(const ConeFwdArgs& a) {
   if (a.bf16) {
     if (a.mode == CF_LINESEARCH) {
-      if (a.w) cone_fwd_kernel_launch<RC, NH, NK, CF_LINESEARCH, true, bf16_bits>(a);
-      else cone_fwd_kernel_launch<RC, NH, NK, CF_LINESEARCH, false, bf16_bits>(a);
+      if (a.w) cone_fwd_kernel_launch<LPC, NH, NK, CF_LINESEARCH, true, bf16_bits>(a);
+      else cone_fwd_kernel_launch<LPC, NH, NK, CF_LINESEARCH, false, bf16_bits>(a);
     } else {
-      cone_fwd_kernel_launch<RC, NH, NK, CF_UPDATE, false, bf16_bits>(a);
+      cone_fwd_kernel_launch<LPC, NH, NK, CF_UPDATE, false, bf16_bits>(a);
     }
   } else {
     switch (a.mode) {
-      case CF_WRITE_VCR: cone_fwd_kernel_launch<RC, NH, NK, CF_WRITE_VCR, false, float>(a); break;
-      case CF_WRITE_VRC: cone_fwd_kernel_launch<RC, NH, NK, CF_WRITE_VRC, false, float>(a); break;
+      case CF_WRITE_VCR: cone_fwd_kernel_launch<LPC, NH, NK, CF_WRITE_VCR, false, float>(a); break;
+      case CF_WRITE_VRC: cone_fwd_kernel_launch<LPC, NH, NK, CF_WRITE_VRC, false, float>(a); break;
       case CF_LINESEARCH:
-        if (a.w) cone_fwd_kernel_launch<RC, NH, NK, CF_LINESEARCH, true, float>(a);
-        else cone_fwd_kernel_launch<RC, NH, NK, CF_LINESEARCH, false, float>(a);
+        if (a.w) cone_fwd_kernel_launch<LPC, NH, NK, CF_LINESEARCH, true, float>(a);
+        else cone_fwd_kernel_launch<LPC, NH, NK, CF_LINESEARCH, false, float>(a);
         break;
-      default: cone_fwd_kernel_launch<RC, NH, NK, CF_UPDATE, false, float>(a);
+      default: cone_fwd_kernel_launch<LPC, NH, NK, CF_UPDATE, false, float>(a);
     }
   }
   MBIR_CHECK_LAUNCH();
@@ -453,14 +566,12 @@ static int cone_fwd_launch3(const ConeFwdArgs& a) {
 
 template <int NH, int NK>
 static int cone_fwd_rc_sw(const ConeFwdArgs& a) {
-  switch (cone_fwd_rc(&a.g)) {
+  switch (cone_fwd_lpc(a.g.num_det_rows)) {
     case 64: return cone_fwd_launch3<64, NH, NK>(a);
     case 32: return cone_fwd_launch3<32, NH, NK>(a);
     case 16: return cone_fwd_launch3<16, NH, NK>(a);
     case 8: return cone_fwd_launch3<8, NH, NK>(a);
-    case 4: return cone_fwd_launch3<4, NH, NK>(a);
-    case 2: return cone_fwd_launch3<2, NH, NK>(a);
-    default: return cone_fwd_launch3<1, NH, NK>(a);
+    default: return cone_fwd_launch3<4, NH, NK>(a);
   }
 }
 
@@ -534,9 +645,10 @@ int mbir_cone_forward(const float* values, const int64_t* pix, int32_t P,
 }
 
 int32_t mbir_cone_fwd_partials_count(const mbir_cone_geom* g) {
-  if (!g) return 0;
-  const int rc = cone_fwd_rc(g);
-  return g->num_views * ((g->num_det_rows + rc - 1) / rc);
+  if (!g || check_cone(g)) return 0;
+  const int lpc = cone_fwd_lpc(g->num_det_rows);
+  return g->num_views * ((g->num_channels + kConeCB - 1) / kConeCB) *
+         ((g->num_det_rows + lpc - 1) / lpc);
 }
 
 static int cone_ls_any(const float* delta, const int64_t* pix, int32_t P, const float* view_table,

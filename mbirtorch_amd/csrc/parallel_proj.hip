@@ -540,28 +540,6 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
 // c0 + u + k*UNITS, k < LPC, accumulating in registers across chunks.  The
 // summation order of every sinogram entry is fixed by the list:
 // deterministic.
-// Rank of this lane's bin b (b < 0: no entry) among the lanes below it with
-// the same bin, the size of its group, and whether it is the group's lowest
-// lane.  One ballot per distinct bin in the wavefront.
-__device__ __forceinline__ void wave_bin_rank(int b, int& rank, int& gsize, bool& leader) {
-  const int lane = threadIdx.x & (MBIR_WAVE - 1);
-  unsigned long long todo = __ballot(b >= 0);
-  rank = 0;
-  gsize = 0;
-  leader = false;
-  while (todo) {
-    const int l = __ffsll(todo) - 1;
-    const int bl = __shfl(b, l, MBIR_WAVE);
-    const unsigned long long m = __ballot(b == bl);
-    if (b == bl) {
-      rank = __popcll(m & ((1ull << lane) - 1ull));
-      gsize = __popcll(m);
-      leader = lane == l;
-    }
-    todo &= ~m;
-  }
-}
-
 constexpr int kShortBin = 24;  // longest bin the insertion sort takes
 
 template <bool TWO>
