@@ -33,9 +33,17 @@ def test_full_size_cone_gradient_matches_finite_differences():
     y = m.forward_project(x0 + 0.01, output_sharded=True).detach()
     x = (x0 + torch.as_tensor(rng.normal(0, 0.01, rs).astype(np.float32), device='cuda'))
     x.requires_grad_(True)
+    import time
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     ax = forward_project_differentiable(m, x)
     loss = ((ax - y) ** 2).sum()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
     loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    print(f'forward {1e3 * (t1 - t0):.1f} ms, backward {1e3 * (t2 - t1):.1f} ms')
     grad = x.grad.detach()
     mask = mb.get_2d_ror_mask(rs)
     rows, cols = np.nonzero(mask)
