@@ -47,6 +47,11 @@ def parse():
                     help='Poisson arrival rate of the serving stream (requests/s)')
     ap.add_argument('--serving-slots', type=int, default=8)
     ap.add_argument('--serving-iters', type=int, default=20)
+    ap.add_argument('--helical', action='store_true',
+                    help='also run the helical mixed-precision configuration (2048 views over '
+                         '4 turns, 64x736 detector, 512x512x128 volume): float32 vs bfloat16 '
+                         'storage VCD')
+    ap.add_argument('--helical-iters', type=int, default=3)
     ap.add_argument('--copy-calibration', action='store_true',
                     help='also run a plain 256 MiB device copy (the reference kernel for '
                          'scaling HBM counters) three times')
@@ -127,6 +132,45 @@ def serving_bench(args):
             'batched': b, 'plain': p,
             'speedup_recons_per_s': b['recons_per_s'] / p['recons_per_s'],
             'max_nrmse_batched_vs_plain': worst}
+
+
+def helical_bench(iters):
+    """Helical cone beam, 2048 views over 4 turns at pitch 1.0, detector 64 x
+    736, magnification 2 (SID 1472, SDD 2944 ALU), volume 512 x 512 x 128
+    (voxel 0.5 ALU, slice aspect 2.5): `iters` VCD iterations with float32 and
+    with bfloat16 storage of the error sinogram from the same seed; time per
+    run, NRMSE between them and the error-sinogram bytes."""
+    import torch
+    import mbirtorch_amd as mb
+    from mbirtorch_amd.phantom import random_ellipsoids, add_noise
+    V, R, C, turns = 2048, 64, 736, 4
+    angles = -np.pi + 2 * np.pi * np.arange(V) / (V // turns)
+    h_iso = R / 2.0
+    z = np.linspace(-turns * h_iso / 2, turns * h_iso / 2, V)
+    out = {'views': V, 'detector': [R, C], 'volume': [512, 512, 128], 'iterations': iters}
+    recons = {}
+    sino = None
+    for storage in ('float32', 'bfloat16'):
+        m = mb.ConeBeamModel((V, R, C), angles, 2944.0, 1472.0, helical_z_shifts=z)
+        m.set_params(recon_shape=(512, 512, 128), delta_voxel=0.5, voxel_slice_aspect=2.5,
+                     sinogram_storage=storage)
+        if sino is None:
+            vol = random_ellipsoids((512, 512, 128), 40, seed=0)
+            sino = add_noise(m.forward_project(vol), 0.01, seed=1)
+        torch.cuda.synchronize()
+        log(f'helical {storage}')
+        t0 = time.perf_counter()
+        np.random.seed(0)
+        recons[storage], d = m.recon(sino, max_iterations=iters, stop_threshold_change_pct=0.0,
+                                     print_logs=False, logfile_path=None, output_sharded=True)
+        torch.cuda.synchronize()
+        out[f'{storage}_s'] = round(time.perf_counter() - t0, 3)
+        out[f'{storage}_fm_rmse'] = d['recon_params']['fm_rmse'][-1]
+    a, b = recons['bfloat16'], recons['float32']
+    out['nrmse_bf16_vs_f32'] = float(torch.sqrt(torch.mean((a - b) ** 2))
+                                     / torch.sqrt(torch.mean(b ** 2)))
+    out['error_sinogram_bytes'] = {'float32': 4 * V * R * C, 'bfloat16': 2 * V * R * C}
+    return out
 
 
 def cpu_baseline(angles, vol, sino, views):
@@ -248,6 +292,7 @@ def main():
     serving = None
     if args.serving_requests > 0:
         serving = serving_bench(args)
+    helical = helical_bench(args.helical_iters) if args.helical else None
 
     cpu = None
     if not args.no_cpu:
@@ -263,7 +308,7 @@ def main():
         'config': {'workload': 'parallel-beam fwd+back: 720 views [0,pi), 64x512 detector, '
                                '512x512x64 volume', 'views': V, 'det_rows': R,
                    'det_channels': C, 'volume': list(rs), 'parallelism': 'single GPU'},
-        'roofline': roofline, 'cpu_baseline': cpu, 'serving': serving,
+        'roofline': roofline, 'cpu_baseline': cpu, 'serving': serving, 'helical_bf16': helical,
         'fwd_ms': round(fwd_ms, 4), 'back_ms': round(back_ms, 4),
         'vcd_iters': None if vcd_ms is None else args.vcd_iters,
         'vcd_ms': None if vcd_ms is None else round(vcd_ms, 1),

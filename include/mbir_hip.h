@@ -167,13 +167,14 @@ int32_t mbir_par_fwd_partials_count(const mbir_par_geom* geom, int32_t ray);
  * search reductions lin = sum(w * e * A delta), quad = sum(w * (A delta)^2)
  * (w == NULL means unit weights), FP64, one (lin, quad) pair per workgroup
  * into partials[2 * mbir_par_fwd_partials_count(geom, row_runs != NULL)].
- * `row_runs` as for mbir_par_forward.  Replaces sparse_forward_project(delta) +
+ * `row_runs` as for mbir_par_forward.  With adelta_vcr non-NULL, A delta is
+ * also stored there (channel-major, float32) for mbir_vcd_error_update.  Replaces sparse_forward_project(delta) +
  * _forward_lin_quad_const/_weighted (tomography_model.py:2592-2610, 65-74). */
 int mbir_par_fwd_linesearch(const float* delta, const int64_t* pixel_indices,
                             int32_t num_pixels, const int32_t* row_runs,
                             const float* view_table, const mbir_par_geom* geom,
                             const float* error_vcr, const float* weights_vcr,
-                            double* partials, mbir_stream_t stream);
+                            float* adelta_vcr, double* partials, mbir_stream_t stream);
 
 /* error -= alpha[0] * (A delta), recomputing A delta in registers (no
  * sinogram-sized transient).  `row_runs` as for mbir_par_forward.  Replaces
@@ -197,8 +198,8 @@ int mbir_par_back_bf16(const uint16_t* sino_vcr, const uint16_t* weights_vcr,
 int mbir_par_fwd_linesearch_bf16(const float* delta, const int64_t* pixel_indices,
                                  int32_t num_pixels, const float* view_table,
                                  const mbir_par_geom* geom, const uint16_t* error_vcr,
-                                 const uint16_t* weights_vcr, double* partials,
-                                 mbir_stream_t stream);
+                                 const uint16_t* weights_vcr, float* adelta_vcr,
+                                 double* partials, mbir_stream_t stream);
 int mbir_par_fwd_update_bf16(const float* delta, const int64_t* pixel_indices,
                              int32_t num_pixels, const float* view_table,
                              const mbir_par_geom* geom, const float* alpha,
@@ -236,7 +237,7 @@ int32_t mbir_cone_fwd_partials_count(const mbir_cone_geom* geom);
 int mbir_cone_fwd_linesearch(const float* delta, const int64_t* pixel_indices,
                              int32_t num_pixels, const float* view_table,
                              const mbir_cone_geom* geom, const float* error_vcr,
-                             const float* weights_vcr, double* partials,
+                             const float* weights_vcr, float* adelta_vcr, double* partials,
                              mbir_stream_t stream);
 
 /* error -= alpha[0] * (A delta) for the cone operator. */
@@ -293,6 +294,14 @@ int mbir_vcd_positivity(const float* flat_recon, const int64_t* pixel_indices,
                         int32_t num_pixels, int32_t num_slices,
                         const float* alpha, float* delta, mbir_stream_t stream);
 
+/* error -= alpha[0] * adelta over n elements (the A delta a line search
+ * stored): the error update of _apply_update (tomography_model.py:77-87) as
+ * one streaming pass.  The bf16 variant rounds the result to nearest even. */
+int mbir_vcd_error_update(float* error_vcr, const float* adelta_vcr, const float* alpha,
+                          int64_t n, mbir_stream_t stream);
+int mbir_vcd_error_update_bf16(uint16_t* error_vcr, const float* adelta_vcr,
+                               const float* alpha, int64_t n, mbir_stream_t stream);
+
 /* Number of doubles of scratch mbir_vcd_apply needs. */
 int64_t mbir_vcd_apply_scratch(int32_t num_pixels, int32_t num_slices);
 
@@ -314,8 +323,8 @@ int mbir_cone_back_bf16(const uint16_t* sino_vcr, const uint16_t* weights_vcr,
 int mbir_cone_fwd_linesearch_bf16(const float* delta, const int64_t* pixel_indices,
                                   int32_t num_pixels, const float* view_table,
                                   const mbir_cone_geom* geom, const uint16_t* error_vcr,
-                                  const uint16_t* weights_vcr, double* partials,
-                                  mbir_stream_t stream);
+                                  const uint16_t* weights_vcr, float* adelta_vcr,
+                                  double* partials, mbir_stream_t stream);
 int mbir_cone_fwd_update_bf16(const float* delta, const int64_t* pixel_indices,
                               int32_t num_pixels, const float* view_table,
                               const mbir_cone_geom* geom, const float* alpha,
@@ -367,6 +376,10 @@ typedef struct {
   double* red_scratch;
   int64_t sino_size;                /* elements of error_vcr */
   int64_t recon_size;               /* elements of flat_recon */
+  float* adelta;                    /* sino_size floats: the line search stores A delta
+                                       and the error update streams it; NULL (or with
+                                       positivity, which clips delta after the line
+                                       search): the update re-projects delta */
 } mbir_vcd_job;
 
 /* Runs the subset steps subsets[0 .. num_steps) of one VCD iteration on

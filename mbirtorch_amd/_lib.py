@@ -87,7 +87,7 @@ class VcdJob(ctypes.Structure):
                 ('fwd_parts', ctypes.c_void_p), ('alpha', ctypes.c_void_p),
                 ('apply_scratch', ctypes.c_void_p), ('dc_scratch', ctypes.c_void_p),
                 ('red_scratch', ctypes.c_void_p), ('sino_size', ctypes.c_int64),
-                ('recon_size', ctypes.c_int64)]
+                ('recon_size', ctypes.c_int64), ('adelta', ctypes.c_void_p)]
 
 
 _P = ctypes.c_void_p
@@ -105,13 +105,13 @@ SIGNATURES = {
     'mbir_par_back': (ctypes.c_int, [_P, _P, _P, _I32, _P, _GP, _I32, _I32, _P, _P]),
     'mbir_reduce_splits': (ctypes.c_int, [_P, _I32, _I64, _P, _P]),
     'mbir_par_fwd_partials_count': (_I32, [_GP, _I32]),
-    'mbir_par_fwd_linesearch': (ctypes.c_int, [_P, _P, _I32, _P, _P, _GP, _P, _P, _P, _P]),
+    'mbir_par_fwd_linesearch': (ctypes.c_int, [_P, _P, _I32, _P, _P, _GP, _P, _P, _P, _P, _P]),
     'mbir_par_fwd_update': (ctypes.c_int, [_P, _P, _I32, _P, _P, _GP, _P, _P, _P]),
     'mbir_cone_forward': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _I32, _P, _P]),
     'mbir_cone_back_splits': (_I32, [_CP, _I32]),
     'mbir_cone_back': (ctypes.c_int, [_P, _P, _P, _I32, _P, _CP, _I32, _I32, _P, _P]),
     'mbir_cone_fwd_partials_count': (_I32, [_CP]),
-    'mbir_cone_fwd_linesearch': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _P, _P, _P, _P]),
+    'mbir_cone_fwd_linesearch': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _P, _P, _P, _P, _P]),
     'mbir_cone_fwd_update': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _P, _P, _P]),
     'mbir_vcd_direction_partials_count': (_I32, [_I32, _I32]),
     'mbir_vcd_direction_dc_scratch': (_I64, [_I32, _I32]),
@@ -121,11 +121,13 @@ SIGNATURES = {
     'mbir_vcd_alpha': (ctypes.c_int, [_P, _I32, _P, _I32, _VP, _P, _P, _P]),
     'mbir_vcd_positivity': (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P]),
     'mbir_vcd_apply_scratch': (ctypes.c_int64, [_I32, _I32]),
+    'mbir_vcd_error_update': (ctypes.c_int, [_P, _P, _P, _I64, _P]),
+    'mbir_vcd_error_update_bf16': (ctypes.c_int, [_P, _P, _P, _I64, _P]),
     'mbir_par_back_bf16': (ctypes.c_int, [_P, _P, _P, _I32, _P, _GP, _I32, _P, _P]),
-    'mbir_par_fwd_linesearch_bf16': (ctypes.c_int, [_P, _P, _I32, _P, _GP, _P, _P, _P, _P]),
+    'mbir_par_fwd_linesearch_bf16': (ctypes.c_int, [_P, _P, _I32, _P, _GP, _P, _P, _P, _P, _P]),
     'mbir_par_fwd_update_bf16': (ctypes.c_int, [_P, _P, _I32, _P, _GP, _P, _P, _P]),
     'mbir_cone_back_bf16': (ctypes.c_int, [_P, _P, _P, _I32, _P, _CP, _I32, _P, _P]),
-    'mbir_cone_fwd_linesearch_bf16': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _P, _P, _P, _P]),
+    'mbir_cone_fwd_linesearch_bf16': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _P, _P, _P, _P, _P]),
     'mbir_cone_fwd_update_bf16': (ctypes.c_int, [_P, _P, _I32, _P, _CP, _P, _P, _P]),
     'mbir_sumsq_bf16': (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
     'mbir_f32_to_bf16': (ctypes.c_int, [_P, _P, _I64, _P]),

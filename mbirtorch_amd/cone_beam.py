@@ -136,11 +136,11 @@ class ConeProjector:
                                                  _lib.stream_ptr()), 'reduce splits')
         return out
 
-    def fwd_linesearch(self, delta, idx, e_vcr, w_vcr, partials, runs=None):
+    def fwd_linesearch(self, delta, idx, e_vcr, w_vcr, partials, runs=None, adelta=None):
         del runs  # one cone forward form
         _lib.check(_lib.lib().mbir_cone_fwd_linesearch(
             _lib.ptr(delta), _lib.ptr(idx), int(idx.shape[0]), _lib.ptr(self.table),
-            self.geom, _lib.ptr(e_vcr), _lib.ptr(w_vcr), _lib.ptr(partials),
+            self.geom, _lib.ptr(e_vcr), _lib.ptr(w_vcr), _lib.ptr(adelta), _lib.ptr(partials),
             _lib.stream_ptr()), 'cone forward line search')
 
     def fwd_update(self, delta, idx, alpha, e_vcr, runs=None):

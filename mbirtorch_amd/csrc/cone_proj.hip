@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_bin_kernel(
     const float* __restrict__ values, const int64_t* __restrict__ pix, int n_pix,
     const ConeView* __restrict__ views, mbir_cone_geom g, TS* __restrict__ sino,
     const TS* __restrict__ wts, const float* __restrict__ alpha,
-    double* __restrict__ partials) {
+    double* __restrict__ partials, float* __restrict__ adelta) {
   using Cfg = ConeBinCfg<NH>;
   constexpr int SPW = MBIR_WAVE / LPC;
   constexpr int UNITS = kCBlock / LPC;
@@ -419,6 +419,7 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_bin_kernel(
       if (MODE == CF_WRITE_VCR) {
         if constexpr (sizeof(TS) == sizeof(float)) sino[o] = t;
       } else if (MODE == CF_LINESEARCH) {
+        if (adelta) adelta[o] = t;  // A(delta) kept for the error update
         const float e = ld_s(sino + o);
         if (WEIGHTED) {
           const float w = ld_s(wts + o);
@@ -529,6 +530,7 @@ struct ConeFwdArgs {
   double* partials;
   hipStream_t s;
   bool bf16;
+  float* adelta;  // line search: also store A(delta) (channel-major), or NULL
 };
 
 template <int LPC, int NH, int NK, int MODE, bool WEIGHTED, typename TS>
@@ -537,7 +539,7 @@ static void cone_fwd_kernel_launch(const ConeFwdArgs& a) {
             (a.g.num_det_rows + LPC - 1) / LPC);
   cone_fwd_bin_kernel<LPC, NH, NK, MODE, WEIGHTED, TS><<<grid, kCBlock, 0, a.s>>>(
       a.values, a.pix, a.P, a.views, a.g, static_cast<TS*>(a.sino),
-      static_cast<const TS*>(a.w), a.alpha, a.partials);
+      static_cast<const TS*>(a.w), a.alpha, a.partials, a.adelta);
 }
 
 template <int LPC, int NH, int NK>
@@ -640,7 +642,7 @@ int mbir_cone_forward(const float* values, const int64_t* pix, int32_t P,
   if (layout != MBIR_LAYOUT_VRC && layout != MBIR_LAYOUT_VCR) return MBIR_ERR_ARG;
   ConeFwdArgs a{layout == MBIR_LAYOUT_VRC ? CF_WRITE_VRC : CF_WRITE_VCR, values, pix, P,
                 reinterpret_cast<const ConeView*>(view_table), *g, out, nullptr, nullptr,
-                nullptr, (hipStream_t)stream, false};
+                nullptr, (hipStream_t)stream, false, nullptr};
   return cone_fwd(a);
 }
 
@@ -653,12 +655,13 @@ int32_t mbir_cone_fwd_partials_count(const mbir_cone_geom* g) {
 
 static int cone_ls_any(const float* delta, const int64_t* pix, int32_t P, const float* view_table,
                        const mbir_cone_geom* g, const void* error_vcr, const void* w,
-                       double* partials, mbir_stream_t stream, bool bf16) {
+                       float* adelta, double* partials, mbir_stream_t stream, bool bf16) {
   int st = check_cone(g);
   if (st) return st;
   if (!delta || !pix || !view_table || !error_vcr || !partials || P < 0) return MBIR_ERR_ARG;
   ConeFwdArgs a{CF_LINESEARCH, delta, pix, P, reinterpret_cast<const ConeView*>(view_table), *g,
-                const_cast<void*>(error_vcr), w, nullptr, partials, (hipStream_t)stream, bf16};
+                const_cast<void*>(error_vcr), w, nullptr, partials, (hipStream_t)stream, bf16,
+                adelta};
   return cone_fwd(a);
 }
 
@@ -669,22 +672,24 @@ static int cone_update_any(const float* delta, const int64_t* pix, int32_t P,
   if (st) return st;
   if (!delta || !pix || !view_table || !error_vcr || !alpha || P < 0) return MBIR_ERR_ARG;
   ConeFwdArgs a{CF_UPDATE, delta, pix, P, reinterpret_cast<const ConeView*>(view_table), *g,
-                error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream, bf16};
+                error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream, bf16, nullptr};
   return cone_fwd(a);
 }
 
 int mbir_cone_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
                              const float* view_table, const mbir_cone_geom* g,
-                             const float* error_vcr, const float* w, double* partials,
-                             mbir_stream_t stream) {
-  return cone_ls_any(delta, pix, P, view_table, g, error_vcr, w, partials, stream, false);
+                             const float* error_vcr, const float* w, float* adelta_vcr,
+                             double* partials, mbir_stream_t stream) {
+  return cone_ls_any(delta, pix, P, view_table, g, error_vcr, w, adelta_vcr, partials, stream,
+                     false);
 }
 
 int mbir_cone_fwd_linesearch_bf16(const float* delta, const int64_t* pix, int32_t P,
                                   const float* view_table, const mbir_cone_geom* g,
-                                  const uint16_t* error_vcr, const uint16_t* w, double* partials,
-                                  mbir_stream_t stream) {
-  return cone_ls_any(delta, pix, P, view_table, g, error_vcr, w, partials, stream, true);
+                                  const uint16_t* error_vcr, const uint16_t* w,
+                                  float* adelta_vcr, double* partials, mbir_stream_t stream) {
+  return cone_ls_any(delta, pix, P, view_table, g, error_vcr, w, adelta_vcr, partials, stream,
+                     true);
 }
 
 int mbir_cone_fwd_update(const float* delta, const int64_t* pix, int32_t P,

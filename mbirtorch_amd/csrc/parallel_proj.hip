@@ -305,10 +305,12 @@ template <int VR, int MODE, bool WEIGHTED, typename TS>
 __device__ __forceinline__ void fwd_epilogue_vcr(const float (&t)[VR], size_t o,
                                                  TS* __restrict__ sino,
                                                  const TS* __restrict__ wts, float al,
+                                                 float* __restrict__ adelta,
                                                  double& lin, double& quad) {
   if (MODE == FWD_WRITE_VCR) {
     if constexpr (sizeof(TS) == sizeof(float)) store_vr<VR>(sino + o, t);
   } else if (MODE == FWD_LINESEARCH) {
+    if (adelta) store_vr<VR>(adelta + o, t);  // A(delta) kept for the error update
     float e[VR];
     load_vr<VR>(sino + o, e);
     if (WEIGHTED) {
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
     const float* __restrict__ values, const int* __restrict__ runs,
     const ParView* __restrict__ views, mbir_par_geom g, float* __restrict__ sino,
     const float* __restrict__ wts, const float* __restrict__ alpha,
-    double* __restrict__ partials) {
+    double* __restrict__ partials, float* __restrict__ adelta) {
   using L = FwdLanes<LPC, VR>;
   constexpr int CHW = L::UNITS * CPS;
   constexpr int TS = CHW + 1;
@@ -525,7 +527,7 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
         const int n = n0 + k;
         if (n < C)
           fwd_epilogue_vcr<VR, MODE, WEIGHTED>(acc[k], ((size_t)v * C + n) * R + rr, sino, wts,
-                                               al, lin, quad);
+                                               al, adelta, lin, quad);
       }
     }
     if (MODE == FWD_LINESEARCH) fwd_write_partials(lin, quad, partials);
@@ -571,7 +573,7 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
     const float* __restrict__ values, const int64_t* __restrict__ pix, int n_pix,
     const ParView* __restrict__ views, mbir_par_geom g, TS* __restrict__ sino,
     const TS* __restrict__ wts, const float* __restrict__ alpha,
-    double* __restrict__ partials) {
+    double* __restrict__ partials, float* __restrict__ adelta) {
   using L = FwdLanes<LPC, VR>;
   using Cfg = BinCfg<TWO>;
   constexpr int NT = Cfg::NT;
@@ -740,7 +742,7 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
         }
       } else {
         fwd_epilogue_vcr<VR, MODE, WEIGHTED>(acc[k], ((size_t)v * C + n) * R + rr, sino, wts,
-                                             al, lin, quad);
+                                             al, adelta, lin, quad);
       }
     }
   }
@@ -828,7 +830,8 @@ struct FwdArgs {
   const float* alpha;
   double* partials;
   hipStream_t s;
-  bool bf16;  // sinogram / weights stored as bfloat16 (binned line search / update)
+  bool bf16;      // sinogram / weights stored as bfloat16 (binned line search / update)
+  float* adelta;  // line search: also store A(delta) (channel-major), or NULL
 };
 
 // Lane configurations (lanes per unit, rows per lane).  VR = 4 or 8 rows per
@@ -872,12 +875,12 @@ static void launch_fwd_kernel(const FwdArgs& a) {
     dim3 grid((g.num_channels + CHW - 1) / CHW, g.num_views, z);
     par_fwd_ray_kernel<LPC, VR, CPS, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
         a.values, a.runs, a.views, g, static_cast<float*>(a.sino),
-        static_cast<const float*>(a.w), a.alpha, a.partials);
+        static_cast<const float*>(a.w), a.alpha, a.partials, a.adelta);
   } else {
     dim3 grid(g.num_views, (g.num_channels + kBlock - 1) / kBlock, z);
     par_fwd_bin_kernel<LPC, VR, TWO, MODE, WEIGHTED, float><<<grid, kBlock, 0, a.s>>>(
         a.values, a.pix, a.P, a.views, g, static_cast<float*>(a.sino),
-        static_cast<const float*>(a.w), a.alpha, a.partials);
+        static_cast<const float*>(a.w), a.alpha, a.partials, a.adelta);
   }
 }
 
@@ -889,7 +892,7 @@ static void launch_fwd_bf16(const FwdArgs& a) {
             (g.num_slices + ROWS - 1) / ROWS);
   par_fwd_bin_kernel<LPC, VR, TWO, MODE, WEIGHTED, bf16_bits><<<grid, kBlock, 0, a.s>>>(
       a.values, a.pix, a.P, a.views, g, static_cast<bf16_bits*>(a.sino),
-      static_cast<const bf16_bits*>(a.w), a.alpha, a.partials);
+      static_cast<const bf16_bits*>(a.w), a.alpha, a.partials, a.adelta);
 }
 
 template <int LPC, int VR, bool TWO>
@@ -1020,7 +1023,7 @@ int mbir_par_forward(const float* values, const int64_t* pix, int32_t P, const i
   if (layout != MBIR_LAYOUT_VRC && layout != MBIR_LAYOUT_VCR) return MBIR_ERR_ARG;
   FwdArgs a{layout == MBIR_LAYOUT_VRC ? FWD_WRITE_VRC : FWD_WRITE_VCR,
             values, pix, P, runs, reinterpret_cast<const ParView*>(view_table), *g, out,
-            nullptr, nullptr, nullptr, (hipStream_t)stream, false};
+            nullptr, nullptr, nullptr, (hipStream_t)stream, false, nullptr};
   return launch_fwd(a);
 }
 
@@ -1032,14 +1035,14 @@ int32_t mbir_par_fwd_partials_count(const mbir_par_geom* g, int32_t ray) {
 int mbir_par_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
                             const int32_t* runs, const float* view_table,
                             const mbir_par_geom* g, const float* error_vcr, const float* w,
-                            double* partials, mbir_stream_t stream) {
+                            float* adelta_vcr, double* partials, mbir_stream_t stream) {
   int st = check_geom(g);
   if (st) return st;
   if (!delta || !view_table || !error_vcr || !partials || P < 0 || (!pix && !runs))
     return MBIR_ERR_ARG;
   FwdArgs a{FWD_LINESEARCH, delta, pix, P, runs,
             reinterpret_cast<const ParView*>(view_table), *g, const_cast<float*>(error_vcr), w,
-            nullptr, partials, (hipStream_t)stream, false};
+            nullptr, partials, (hipStream_t)stream, false, adelta_vcr};
   return launch_fwd(a);
 }
 
@@ -1051,20 +1054,20 @@ int mbir_par_fwd_update(const float* delta, const int64_t* pix, int32_t P, const
   if (!delta || !view_table || !error_vcr || !alpha || P < 0 || (!pix && !runs))
     return MBIR_ERR_ARG;
   FwdArgs a{FWD_UPDATE, delta, pix, P, runs, reinterpret_cast<const ParView*>(view_table), *g,
-            error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream, false};
+            error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream, false, nullptr};
   return launch_fwd(a);
 }
 
 int mbir_par_fwd_linesearch_bf16(const float* delta, const int64_t* pix, int32_t P,
                                  const float* view_table, const mbir_par_geom* g,
-                                 const uint16_t* error_vcr, const uint16_t* w, double* partials,
-                                 mbir_stream_t stream) {
+                                 const uint16_t* error_vcr, const uint16_t* w, float* adelta_vcr,
+                                 double* partials, mbir_stream_t stream) {
   int st = check_geom(g);
   if (st) return st;
   if (!delta || !pix || !view_table || !error_vcr || !partials || P < 0) return MBIR_ERR_ARG;
   FwdArgs a{FWD_LINESEARCH, delta, pix, P, nullptr,
             reinterpret_cast<const ParView*>(view_table), *g, const_cast<uint16_t*>(error_vcr),
-            w, nullptr, partials, (hipStream_t)stream, true};
+            w, nullptr, partials, (hipStream_t)stream, true, adelta_vcr};
   return launch_fwd(a);
 }
 
@@ -1075,7 +1078,7 @@ int mbir_par_fwd_update_bf16(const float* delta, const int64_t* pix, int32_t P,
   if (st) return st;
   if (!delta || !pix || !view_table || !error_vcr || !alpha || P < 0) return MBIR_ERR_ARG;
   FwdArgs a{FWD_UPDATE, delta, pix, P, nullptr, reinterpret_cast<const ParView*>(view_table),
-            *g, error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream, true};
+            *g, error_vcr, nullptr, alpha, nullptr, (hipStream_t)stream, true, nullptr};
   return launch_fwd(a);
 }
 

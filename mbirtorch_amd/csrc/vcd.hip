@@ -239,6 +239,18 @@ static inline int apply_blocks(int P) {
   return nb > 1024 ? 1024 : (nb < 1 ? 1 : nb);
 }
 
+// e -= alpha * A(delta), streaming over the stored projection (the same
+// float32 operations as the projector's update epilogue).
+template <typename TS>
+__global__ __launch_bounds__(kThreads) void error_update_kernel(
+    TS* __restrict__ e, const float* __restrict__ adelta, const float* __restrict__ alpha,
+    int64_t n) {
+  const float al = alpha[0];
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+    st_s(e + i, ld_s(e + i) - al * adelta[i]);
+}
+
 // Storage conversion (float32 <-> bfloat16, round to nearest even).
 template <typename TI, typename TO>
 __global__ __launch_bounds__(kThreads) void convert_kernel(const TI* __restrict__ in,
@@ -441,6 +453,31 @@ int mbir_sumsq_bf16(const uint16_t* a, const uint16_t* w, int64_t n, double* scr
   sumsq_partial_kernel<bf16_bits><<<nb, kThreads, 0, s>>>(a, w, n, scratch);
   MBIR_CHECK_LAUNCH();
   final3_kernel<<<1, kThreads, 0, s>>>(scratch, nb, 3, out);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+static int error_update_grid(int64_t n) {
+  int64_t nb = (n + kThreads - 1) / kThreads;
+  return (int)(nb > 16384 ? 16384 : (nb < 1 ? 1 : nb));
+}
+
+int mbir_vcd_error_update(float* error_vcr, const float* adelta_vcr, const float* alpha,
+                          int64_t n, mbir_stream_t stream) {
+  if (!error_vcr || !adelta_vcr || !alpha || n < 0) return MBIR_ERR_ARG;
+  if (n == 0) return MBIR_OK;
+  error_update_kernel<float><<<error_update_grid(n), kThreads, 0, (hipStream_t)stream>>>(
+      error_vcr, adelta_vcr, alpha, n);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int mbir_vcd_error_update_bf16(uint16_t* error_vcr, const float* adelta_vcr, const float* alpha,
+                               int64_t n, mbir_stream_t stream) {
+  if (!error_vcr || !adelta_vcr || !alpha || n < 0) return MBIR_ERR_ARG;
+  if (n == 0) return MBIR_OK;
+  error_update_kernel<bf16_bits><<<error_update_grid(n), kThreads, 0, (hipStream_t)stream>>>(
+      error_vcr, adelta_vcr, alpha, n);
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }

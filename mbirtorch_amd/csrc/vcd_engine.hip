@@ -73,27 +73,35 @@ int subset_step(const mbir_vcd_job* j, int k, double* stats, hipStream_t s) {
     MBIR_TRY(mbir_vcd_direction(j->flat_recon, pix, P, j->back_parts, ns, j->hessian, j->prox,
                                 &j->prm, j->delta, j->dir_parts, s));
   int n_fwd;
+  float* adelta = j->positivity ? nullptr : j->adelta;
   if (par) {
     if (h)
       MBIR_TRY(mbir_par_fwd_linesearch_bf16(j->delta, pix, P, j->view_table, j->par_geom, e16,
-                                            w16, j->fwd_parts, s));
+                                            w16, adelta, j->fwd_parts, s));
     else
       MBIR_TRY(mbir_par_fwd_linesearch(j->delta, pix, P, runs, j->view_table, j->par_geom,
-                                       j->error_vcr, j->weights_vcr, j->fwd_parts, s));
+                                       j->error_vcr, j->weights_vcr, adelta, j->fwd_parts, s));
     n_fwd = mbir_par_fwd_partials_count(j->par_geom, runs != nullptr);
   } else {
     if (h)
       MBIR_TRY(mbir_cone_fwd_linesearch_bf16(j->delta, pix, P, j->view_table, j->cone_geom, e16,
-                                             w16, j->fwd_parts, s));
+                                             w16, adelta, j->fwd_parts, s));
     else
       MBIR_TRY(mbir_cone_fwd_linesearch(j->delta, pix, P, j->view_table, j->cone_geom,
-                                        j->error_vcr, j->weights_vcr, j->fwd_parts, s));
+                                        j->error_vcr, j->weights_vcr, adelta, j->fwd_parts, s));
     n_fwd = mbir_cone_fwd_partials_count(j->cone_geom);
   }
   MBIR_TRY(mbir_vcd_alpha(j->dir_parts, n_dir, j->fwd_parts, n_fwd, &j->prm, j->alpha,
                           stats + S + 1, s));
   if (j->positivity) MBIR_TRY(mbir_vcd_positivity(j->flat_recon, pix, P, S, j->alpha, j->delta, s));
-  if (par && h)
+  // The positivity clip changes delta after the line search, so A(delta)
+  // is re-projected then; otherwise the stored A(delta) is streamed.
+  const bool stored = j->adelta && !j->positivity;
+  if (stored && h)
+    MBIR_TRY(mbir_vcd_error_update_bf16(e16w, j->adelta, j->alpha, j->sino_size, s));
+  else if (stored)
+    MBIR_TRY(mbir_vcd_error_update(j->error_vcr, j->adelta, j->alpha, j->sino_size, s));
+  else if (par && h)
     MBIR_TRY(mbir_par_fwd_update_bf16(j->delta, pix, P, j->view_table, j->par_geom, j->alpha,
                                       e16w, s));
   else if (par)

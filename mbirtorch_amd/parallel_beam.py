@@ -147,13 +147,15 @@ class ParallelProjector:
                                                  _lib.stream_ptr()), 'reduce splits')
         return out
 
-    def fwd_linesearch(self, delta, idx, e_vcr, w_vcr, partials, runs=None):
+    def fwd_linesearch(self, delta, idx, e_vcr, w_vcr, partials, runs=None, adelta=None):
         """FP64 line-search partials of A(delta) against the error sinogram;
-        ``runs`` (row_runs of ``idx``, or None) selects the ray-driven form."""
+        ``runs`` (row_runs of ``idx``, or None) selects the ray-driven form;
+        ``adelta`` (channel-major float32 sinogram, or None) receives A(delta)."""
         _lib.check(_lib.lib().mbir_par_fwd_linesearch(
             _lib.ptr(delta), _lib.ptr(idx), int(idx.shape[0]), _lib.ptr(runs),
             _lib.ptr(self.table), self.geom, _lib.ptr(e_vcr), _lib.ptr(w_vcr),
-            _lib.ptr(partials), _lib.stream_ptr()), 'parallel forward line search')
+            _lib.ptr(adelta), _lib.ptr(partials), _lib.stream_ptr()),
+            'parallel forward line search')
 
     def fwd_update(self, delta, idx, alpha, e_vcr, runs=None):
         _lib.check(_lib.lib().mbir_par_fwd_update(

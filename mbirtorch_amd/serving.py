@@ -39,7 +39,7 @@ def request_device_bytes(sino_shape, recon_shape, granularity, weighted=False):
     the request is prepared (measured sinogram + filtered sinogram + Hessian
     weights + the full-region cylinders).  step: VCD scratch of the largest
     subset (back-projection partials for up to one split per 8 views,
-    direction, partials).
+    direction, partials) plus the stored A(delta) (one float32 sinogram).
     """
     V, R, C = (int(v) for v in sino_shape)
     Nr, Nc, S = (int(v) for v in recon_shape)
@@ -56,7 +56,7 @@ def request_device_bytes(sino_shape, recon_shape, granularity, weighted=False):
     resident = sino + 2 * vol + (sino if weighted else 0) + parts + 8 * 64 * (S + 6)
     init = 3 * sino + 4 * P * S
     splits = max(1, V // 8)
-    step = 4 * max_p * S * (splits + 2) + 8 * 2 * (V * C + 4 * max_p * S // 256 + 4096)
+    step = (4 * max_p * S * (splits + 2) + sino + 8 * 2 * (V * C + 4 * max_p * S // 256 + 4096))
     return {'resident': resident, 'init': init, 'step': step,
             'total': resident + max(init, step)}
 

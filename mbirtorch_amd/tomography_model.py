@@ -765,6 +765,9 @@ class _VcdEngine:
         self.alpha = torch.zeros(1, dtype=torch.float32, device=dev)
         n_apply = int(lib.mbir_vcd_apply_scratch(max_p, S))
         self.apply_scratch = torch.empty(max(n_apply, 1), dtype=torch.float64, device=dev)
+        # A(delta) of the current subset, kept by the line search so the error
+        # update is one streaming pass (sinogram-sized float32).
+        self.adelta = torch.empty(e.numel(), dtype=torch.float32, device=dev)
         n_red = 3 * int(lib.mbir_reduce_blocks(max(e.numel(), x.numel())))
         self.red_scratch = torch.empty(n_red, dtype=torch.float64, device=dev)
         self.profile = model._direction_profile()
@@ -807,6 +810,7 @@ class _VcdEngine:
         job.red_scratch = self.red_scratch.data_ptr()
         job.sino_size = e.numel()
         job.recon_size = x.numel()
+        job.adelta = self.adelta.data_ptr()
         self.job = job
 
     def iteration(self, pidx, order, total_sino):

@@ -155,21 +155,19 @@ def system_matrix(pix, g, coeff_power=1):
                 continue
             mk = m0[v, j] + Wr[v, j] * kk                    # (S,)
             hw_r = int(np.ceil((Wr[v, j] + 1) / 2)) + 1
-            for k in range(ns):
-                mc = int(np.floor(mk[k]))
-                mm = np.arange(mc - hw_r, mc + hw_r + 1)
-                av = _trap(mk[k] - mm, Wr[v, j]) * inv_cos[k]
-                kp = (av > 0) & (mm >= 0) & (mm < R)
-                mm, av = mm[kp], av[kp]
-                if mm.size == 0:
-                    continue
-                w = (av[:, None] * ah[None, :])
-                if coeff_power != 1:
-                    w = w ** coeff_power
-                rr = (v * R + mm)[:, None] * C + nn[None, :]
-                rows.append(rr.ravel())
-                cols.append(np.full(rr.size, j * ns + k))
-                vals.append(w.ravel())
+            # all slices at once: rows floor(m(k)) - hw_r .. + hw_r
+            mm = np.floor(mk).astype(np.int64)[:, None] + np.arange(-hw_r, hw_r + 1)[None, :]
+            av = _trap(mk[:, None] - mm, Wr[v, j]) * inv_cos[:, None]
+            ks, ts = np.nonzero((av > 0) & (mm >= 0) & (mm < R))
+            if ks.size == 0:
+                continue
+            w = av[ks, ts][:, None] * ah[None, :]
+            if coeff_power != 1:
+                w = w ** coeff_power
+            rr = (v * R + mm[ks, ts])[:, None] * C + nn[None, :]
+            rows.append(rr.ravel())
+            cols.append(np.broadcast_to((j * ns + ks)[:, None], rr.shape).ravel())
+            vals.append(w.ravel())
     rows = np.concatenate(rows) if rows else np.zeros(0, np.int64)
     cols = np.concatenate(cols) if cols else np.zeros(0, np.int64)
     vals = np.concatenate(vals) if vals else np.zeros(0)

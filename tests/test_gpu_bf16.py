@@ -72,7 +72,7 @@ def test_parallel_bf16_kernels_equal_f32_on_widened_values(weighted):
     ops.fwd_linesearch(delta, idx, e32, w32, p32)
     _lib.check(lib.mbir_par_fwd_linesearch_bf16(_lib.ptr(delta), _lib.ptr(idx), P,
                                                 _lib.ptr(ops.table), ops.geom, _lib.ptr(e16),
-                                                _lib.ptr(w16), _lib.ptr(p16), st), 'ls')
+                                                _lib.ptr(w16), None, _lib.ptr(p16), st), 'ls')
     assert torch.equal(p16, p32)
     alpha = torch.full((1,), 0.37, device='cuda')
     ops.fwd_update(delta, idx, alpha, e32)
