@@ -83,6 +83,19 @@ __device__ __forceinline__ void block_sum2_f64(double& a, double& b) {
   }
 }
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the
+// MI355X's 8 XCDs (linear id mod 8), each with its own L2.  xcd_remap maps
+// the linear dispatch id to a logical id so that XCD x runs one contiguous
+// range of logical ids -- neighbouring tiles (which read the same sinogram
+// rows or voxel cylinders) then share an L2.  A bijection on [0, n).
+constexpr int kNumXcd = 8;
+
+__device__ __forceinline__ int xcd_remap(int id, int n) {
+  const int per = n / kNumXcd, rem = n % kNumXcd;
+  const int xcd = id % kNumXcd, k = id / kNumXcd;
+  return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
 // Rank of this lane's bin b (b < 0: no entry) among the lanes below it with
 // the same bin, the size of its group, and whether it is the group's lowest
 // lane.  One ballot per distinct bin in the wavefront.

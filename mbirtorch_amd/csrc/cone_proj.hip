@@ -112,12 +112,13 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
   const int lane = threadIdx.x & (MBIR_WAVE - 1), wave = threadIdx.x / MBIR_WAVE;
   const int slot = lane / RW, kl = lane % RW;
   const int S = g.num_slices, C = g.num_channels, R = g.num_det_rows;
-  const int k = blockIdx.z * RW + kl;
+  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  const int k = bz * RW + kl;
   const bool k_ok = k < S;
   const float kf = (float)k;
   const float zrel = g.delta_voxel_slice * (kf - g.slice_center);
-  const int p_base = blockIdx.x * PIX_WG;
-  const int v_begin = blockIdx.y * views_per_split;
+  const int p_base = bx * PIX_WG;
+  const int v_begin = by * views_per_split;
   const int v_end = min(v_begin + views_per_split, g.num_views);
   constexpr int STRIDE = VB * PIX_WG;
 
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
 #pragma unroll
     for (int i = 0; i < ACC; ++i) {
       const int pg = p_base + wave * PIX_WAVE + i * SLOTS + slot;
-      if (pg < n_pix) out[((size_t)blockIdx.y * n_pix + pg) * S + k] = acc[i];
+      if (pg < n_pix) out[((size_t)by * n_pix + pg) * S + k] = acc[i];
     }
   }
 }

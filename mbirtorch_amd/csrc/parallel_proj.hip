@@ -107,10 +107,11 @@ __global__ __launch_bounds__(kBlock) void par_back_kernel(
   const int lane = threadIdx.x & (MBIR_WAVE - 1), wave = threadIdx.x / MBIR_WAVE;
   const int slot = lane / RW, rl = lane % RW;
   const int R = g.num_slices, C = g.num_channels;
-  const int r = blockIdx.z * RW + rl;
+  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  const int r = bz * RW + rl;
   const bool r_ok = r < R;
-  const int p_base = blockIdx.x * PIX_WG;
-  const int v_begin = blockIdx.y * views_per_split;
+  const int p_base = bx * PIX_WG;
+  const int v_begin = by * views_per_split;
   const int v_end = min(v_begin + views_per_split, g.num_views);
 
   float acc[ACC];
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void par_back_kernel(
 #pragma unroll
     for (int k = 0; k < ACC; ++k) {
       const int pg = p_base + wave * PIX_WAVE + k * SLOTS + slot;
-      if (pg < n_pix) out[((size_t)blockIdx.y * n_pix + pg) * R + r] = acc[k];
+      if (pg < n_pix) out[((size_t)by * n_pix + pg) * R + r] = acc[k];
     }
   }
 }
@@ -380,13 +381,13 @@ __device__ __forceinline__ void fwd_taps(int64_t pix, const ParView& vw, const m
 // (pixel, channel) weight is found and weights agree bit for bit.
 constexpr float kRayMargin = 0.01f;
 
-// Four candidates (i[u], j[u]) at list positions pos[u], those with ok[u]
+// NB candidates (i[u], j[u]) at list positions pos[u], those with ok[u]
 // live, in one batch: geometry first, then all cylinder loads, then the FMAs,
 // so the loads of a batch are in flight together.
-template <int VR, int CPS, bool TWO>
-__device__ __forceinline__ void ray_taps4(const int (&ci)[4], const int (&cj)[4],
-                                          const int (&pos)[4], const bool (&ok)[4], int n0,
-                                          int ntaps,
+template <int NB, int VR, int CPS, bool TWO>
+__device__ __forceinline__ void ray_taps(const int (&ci)[NB], const int (&cj)[NB],
+                                         const int (&pos)[NB], const bool (&ok)[NB], int n0,
+                                         int ntaps,
                                           bool row_ok, int rr, const ParView& vw,
                                           const mbir_par_geom& g,
                                           const float* __restrict__ values,
@@ -394,10 +395,10 @@ __device__ __forceinline__ void ray_taps4(const int (&ci)[4], const int (&cj)[4]
   // The trapezoid weight is zero outside the pixel's taps (its support
   // |n - n_p| < (W+1)/2 lies inside round(n_p) +- psf, and inside floor(n_p),
   // floor(n_p)+1 when W <= 1), so the weight alone selects the taps.
-  float a[4][CPS];
-  bool use[4];
+  float a[NB][CPS];
+  bool use[NB];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
+  for (int u = 0; u < NB; ++u) {
     const float n_p = par_np(ci[u], cj[u], vw, g);
     bool any = false;
 #pragma unroll
@@ -407,9 +408,9 @@ __device__ __forceinline__ void ray_taps4(const int (&ci)[4], const int (&cj)[4]
     }
     use[u] = any && row_ok && ok[u];
   }
-  float x[4][VR];
+  float x[NB][VR];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
+  for (int u = 0; u < NB; ++u) {
     if (use[u]) {
       load_vr<VR>(values + (size_t)pos[u] * g.num_slices + rr, x[u]);
     } else {
@@ -418,14 +419,14 @@ __device__ __forceinline__ void ray_taps4(const int (&ci)[4], const int (&cj)[4]
     }
   }
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int u = 0; u < NB; ++u)
 #pragma unroll
     for (int k = 0; k < CPS; ++k)
 #pragma unroll
       for (int q = 0; q < VR; ++q) acc[k][q] = fmaf(a[u][k], x[u][q], acc[k][q]);
 }
 
-template <int LPC, int VR, int CPS, bool TWO, int MODE, bool WEIGHTED>
+template <int LPC, int VR, int CPS, int NB, bool TWO, int MODE, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
     const float* __restrict__ values, const int* __restrict__ runs,
     const ParView* __restrict__ views, mbir_par_geom g, float* __restrict__ sino,
@@ -438,7 +439,17 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
 
   const int lane = threadIdx.x & (MBIR_WAVE - 1), wave = threadIdx.x / MBIR_WAVE;
   const int unit = wave * L::SPW + lane / LPC, rl = lane % LPC;
-  const int c_base = blockIdx.x * CHW, v = blockIdx.y, r0 = blockIdx.z * L::ROWS;
+  // Logical order: view fastest, so an XCD runs one channel block over
+  // consecutive views (overlapping pixel bands share its L2).
+  int c_base, v, r0;
+  {
+    const int nv = gridDim.y, ncb = gridDim.x;
+    const int l = xcd_remap(blockIdx.x + ncb * (blockIdx.y + nv * blockIdx.z),
+                            ncb * nv * gridDim.z);
+    v = l % nv;
+    c_base = ((l / nv) % ncb) * CHW;
+    r0 = (l / (nv * ncb)) * L::ROWS;
+  }
   const int R = g.num_slices, C = g.num_channels, Nr = g.num_rows, Nc = g.num_cols;
   const int n0 = c_base + unit * CPS;
   const int rr = r0 + rl * VR;
@@ -465,18 +476,18 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
       const float ja = (xa + sy) / cd + g.col_center, jb = (xb + sy) / cd + g.col_center;
       const int jlo = max((int)floorf(fminf(ja, jb) - hj), jmin);
       const int jhi = min((int)ceilf(fmaxf(ja, jb) + hj), jmin + cnt - 1);
-      for (int j = jlo; j <= jhi; j += 4) {
-        int ci[4], cj[4], pos[4];
-        bool ok[4];
+      for (int j = jlo; j <= jhi; j += NB) {
+        int ci[NB], cj[NB], pos[NB];
+        bool ok[NB];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < NB; ++u) {
           const int jj = min(j + u, jhi);
           ci[u] = i;
           cj[u] = jj;
           pos[u] = start + jj - jmin;
           ok[u] = j + u <= jhi;
         }
-        ray_taps4<VR, CPS, TWO>(ci, cj, pos, ok, n0, ntaps, row_ok, rr, vw, g, values, acc);
+        ray_taps<NB, VR, CPS, TWO>(ci, cj, pos, ok, n0, ntaps, row_ok, rr, vw, g, values, acc);
       }
     }
   } else {
@@ -486,11 +497,11 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
       const float ia = (cx - xa) / sd + g.row_center, ib = (cx - xb) / sd + g.row_center;
       const int ilo = max((int)floorf(fminf(ia, ib) - hi), 0);
       const int ihi = min((int)ceilf(fmaxf(ia, ib) + hi), Nr - 1);
-      for (int i = ilo; i <= ihi; i += 4) {
-        int ci[4], cj[4], pos[4];
-        bool ok[4];
+      for (int i = ilo; i <= ihi; i += NB) {
+        int ci[NB], cj[NB], pos[NB];
+        bool ok[NB];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < NB; ++u) {
           const int ii = min(i + u, ihi);
           const int start = runs[3 * ii], jmin = runs[3 * ii + 1], cnt = runs[3 * ii + 2];
           ci[u] = ii;
@@ -498,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
           pos[u] = start + j - jmin;
           ok[u] = i + u <= ihi && j >= jmin && j < jmin + cnt;
         }
-        ray_taps4<VR, CPS, TWO>(ci, cj, pos, ok, n0, ntaps, row_ok, rr, vw, g, values, acc);
+        ray_taps<NB, VR, CPS, TWO>(ci, cj, pos, ok, n0, ntaps, row_ok, rr, vw, g, values, acc);
       }
     }
   }
@@ -845,8 +856,10 @@ static inline LaneCfg bin_lanes(int R, bool vec) {
   if (vec) return {R <= 8 ? 2 : (R <= 16 ? 4 : (R <= 32 ? 8 : 16)), 4};
   return {R <= 16 ? 16 : 64, 1};
 }
-// The ray form shares its geometry over the lanes of a unit, so it takes
-// eight rows per lane (fewer lanes per unit) for tall cylinders.
+// The ray form repeats a candidate's geometry on every lane of its unit, so
+// tall cylinders take eight rows per lane (fewer lanes per unit).  (Thinner
+// units -- two lanes of 32 rows, one channel -- issue fewer VALU
+// instructions but ~1.8x the cylinder loads, and measured 1.7x slower.)
 static inline LaneCfg ray_lanes(int R, bool vec) {
   if (vec && R > 32 && R % 8 == 0) return {8, 8};
   return bin_lanes(R, vec);
@@ -871,9 +884,10 @@ static void launch_fwd_kernel(const FwdArgs& a) {
   const int z = (g.num_slices + ROWS - 1) / ROWS;
   if (a.runs) {
     constexpr int CPS = LPC >= 8 ? 4 : (LPC == 4 ? 2 : 1);
+    constexpr int NB = 4;  // candidates per batch
     constexpr int CHW = (kBlock / LPC) * CPS;
     dim3 grid((g.num_channels + CHW - 1) / CHW, g.num_views, z);
-    par_fwd_ray_kernel<LPC, VR, CPS, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
+    par_fwd_ray_kernel<LPC, VR, CPS, NB, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
         a.values, a.runs, a.views, g, static_cast<float*>(a.sino),
         static_cast<const float*>(a.w), a.alpha, a.partials, a.adelta);
   } else {
