@@ -315,6 +315,39 @@ int mbir_vcd_apply(float* flat_recon, const int64_t* pixel_indices,
                    const float* alpha, double* acc, double* scratch,
                    mbir_stream_t stream);
 
+/* The end of a subset step as one launch: the step size, the error update
+ * error -= alpha*adelta over n_sino elements (n_sino = 0: none), and
+ * x[pix] += alpha*delta with its per-slice statistics.  With dir_parts set,
+ * alpha is formed from the prior and line-search partials exactly as
+ * mbir_vcd_alpha does (written to alpha[0], added to alpha_sum[0]);
+ * dir_parts = NULL reads alpha[0].  The statistics -- sum_p (alpha*delta[p,
+ * s])^2 per slice, then sum |alpha*delta| -- are ADDED into per-workgroup rows
+ * block_stats[b*(num_slices+1) + c], b < mbir_vcd_block_stats_rows(P);
+ * mbir_vcd_block_stats_reduce folds them into an accumulator and clears
+ * them.  block_stats holds mbir_vcd_apply_scratch(max P, num_slices) doubles,
+ * zero before its first use.  Fuses mbir_vcd_alpha, mbir_vcd_error_update and
+ * mbir_vcd_apply (tomography_model.py:2612-2617, 77-87). */
+int mbir_vcd_finish(float* error_vcr, const float* adelta_vcr, int64_t n_sino,
+                    float* flat_recon, const int64_t* pixel_indices, int32_t num_pixels,
+                    int32_t num_slices, const float* delta, const double* dir_parts,
+                    int32_t num_dir, const double* fwd_parts, int32_t num_fwd,
+                    const mbir_vcd_params* prm, float* alpha, double* alpha_sum,
+                    double* block_stats, mbir_stream_t stream);
+int mbir_vcd_finish_bf16(uint16_t* error_vcr, const float* adelta_vcr, int64_t n_sino,
+                         float* flat_recon, const int64_t* pixel_indices,
+                         int32_t num_pixels, int32_t num_slices, const float* delta,
+                         const double* dir_parts, int32_t num_dir, const double* fwd_parts,
+                         int32_t num_fwd, const mbir_vcd_params* prm, float* alpha,
+                         double* alpha_sum, double* block_stats, mbir_stream_t stream);
+
+/* Rows of block_stats that mbir_vcd_finish writes for a subset of P pixels. */
+int32_t mbir_vcd_block_stats_rows(int32_t num_pixels);
+
+/* acc[c] += sum_b block_stats[b*(num_slices+1) + c] for c <= num_slices in a
+ * fixed order over rows b < rows, and clears those rows. */
+int mbir_vcd_block_stats_reduce(double* block_stats, int32_t rows, int32_t num_slices,
+                                double* acc, mbir_stream_t stream);
+
 /* bfloat16-storage cone variants (see the parallel ones). */
 int mbir_cone_back_bf16(const uint16_t* sino_vcr, const uint16_t* weights_vcr,
                         const int64_t* pixel_indices, int32_t num_pixels,
@@ -344,7 +377,9 @@ int mbir_cone_fwd_update_bf16(const float* delta, const int64_t* pixel_indices,
  * floats (max over subsets of splits * size * slices), delta >= max size *
  * slices floats, dir_parts >= 2 * mbir_vcd_direction_partials_count,
  * fwd_parts >= 2 * max forward partials count, apply_scratch >=
- * mbir_vcd_apply_scratch, red_scratch >= 3 * mbir_reduce_blocks(max(
+ * mbir_vcd_apply_scratch(max size, slices) doubles and zero before the first
+ * run (it carries per-workgroup statistics to the end of each iteration,
+ * which leaves it zero), red_scratch >= 3 * mbir_reduce_blocks(max(
  * sino_size, recon_size)), dc_scratch >= mbir_vcd_direction_dc_scratch when
  * dc_profile is set. */
 typedef struct {
@@ -385,8 +420,10 @@ typedef struct {
 /* Runs the subset steps subsets[0 .. num_steps) of one VCD iteration on
  * `stream`, accumulating into stats (num_slices + 6 doubles, zeroed by the
  * caller: per-slice sum (alpha*delta)^2, sum |alpha*delta|, sum alpha, then
- * with end_iteration != 0 the forward-model sums sum w*e^2, sum e^2, sum w
- * and sum |x|).  No host synchronisation.  Replaces the subset loop of
+ * the forward-model sums sum w*e^2, sum e^2, sum w and sum |x|).  The
+ * per-slice sums and sum |alpha*delta| of all steps since the previous
+ * iteration end, and the forward-model sums, land with end_iteration != 0;
+ * sum alpha lands at every step.  No host synchronisation.  Replaces the subset loop of
  * vcd_partition_iterator / _vcd_recon (tomography_model.py:2662-2697,
  * 2851-2960). */
 int mbir_vcd_run(const mbir_vcd_job* job, const int32_t* subsets, int32_t num_steps,

@@ -139,6 +139,12 @@ SIGNATURES = {
     'mbir_server_in_flight': (_I32, [_P]),
     'mbir_server_destroy': (None, [_P]),
     'mbir_vcd_apply': (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P, _P, _P]),
+    'mbir_vcd_finish': (ctypes.c_int, [_P, _P, _I64, _P, _P, _I32, _I32, _P, _P, _I32, _P,
+                                       _I32, _VP, _P, _P, _P, _P]),
+    'mbir_vcd_finish_bf16': (ctypes.c_int, [_P, _P, _I64, _P, _P, _I32, _I32, _P, _P, _I32,
+                                            _P, _I32, _VP, _P, _P, _P, _P]),
+    'mbir_vcd_block_stats_rows': (_I32, [_I32]),
+    'mbir_vcd_block_stats_reduce': (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
     'mbir_reduce_blocks': (_I32, [_I64]),
     'mbir_sumsq_f64': (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
     'mbir_sumabs_f64': (ctypes.c_int, [_P, _I64, _P, _P, _P]),

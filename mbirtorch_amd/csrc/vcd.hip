@@ -63,8 +63,18 @@ __device__ __forceinline__ void vcd_terms(const float* __restrict__ x,
       hp = hp + c2;
     }
   }
+  // The split partials are added in split order; the loads go out in
+  // batches of eight so their latencies overlap.
   float bsum = back_parts[i];
-  for (int k = 1; k < nsplit; ++k) bsum += back_parts[(size_t)k * total + i];
+  int k = 1;
+  for (; k + 8 <= nsplit; k += 8) {
+    float t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = back_parts[(size_t)(k + u) * total + i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) bsum += t[u];
+  }
+  for (; k < nsplit; ++k) bsum += back_parts[(size_t)k * total + i];
   fg = prm.neg_fm_constant * bsum;
   fh = prm.fm_constant * hess[pi * S + s];
 }
@@ -75,15 +85,15 @@ __global__ __launch_bounds__(kThreads) void vcd_direction_kernel(
     const float* __restrict__ prox, mbir_vcd_params prm, float* __restrict__ delta,
     double* __restrict__ partials) {
   const int64_t total = (int64_t)P * prm.num_slices;
-  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   double pl = 0.0, pq = 0.0;
-  if (i < total) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * kThreads) {
     float fg, gp, fh, hp;
     vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp);
     const float d = -((fg + gp) / (fh + hp));
     delta[i] = d;
-    pl = (double)(gp * d);
-    pq = (double)((hp * d) * d);
+    pl += (double)(gp * d);
+    pq += (double)((hp * d) * d);
   }
   block_sum2_f64(pl, pq);
   if (threadIdx.x == 0) {
@@ -98,14 +108,15 @@ __global__ __launch_bounds__(kThreads) void vcd_dc_terms_kernel(
     const float* __restrict__ back_parts, int nsplit, const float* __restrict__ hess,
     const float* __restrict__ prox, mbir_vcd_params prm, float* __restrict__ scratch) {
   const int64_t total = (int64_t)P * prm.num_slices;
-  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (i >= total) return;
-  float fg, gp, fh, hp;
-  vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp);
-  scratch[i] = fg + gp;                  // g
-  scratch[total + i] = 1.0f / (fh + hp); // h_inv
-  scratch[2 * total + i] = gp;
-  scratch[3 * total + i] = hp;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * kThreads) {
+    float fg, gp, fh, hp;
+    vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp);
+    scratch[i] = fg + gp;                   // g
+    scratch[total + i] = 1.0f / (fh + hp);  // h_inv
+    scratch[2 * total + i] = gp;
+    scratch[3 * total + i] = hp;
+  }
 }
 
 // DC damping pass 2: one workgroup per slice, gbar_k in FP64 (fixed order),
@@ -131,17 +142,17 @@ __global__ __launch_bounds__(kThreads) void vcd_dc_delta_kernel(
     const float* __restrict__ prof, int P, int S, float* __restrict__ delta,
     double* __restrict__ partials) {
   const int64_t total = (int64_t)P * S;
-  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   double pl = 0.0, pq = 0.0;
-  if (i < total) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * kThreads) {
     const int s = (int)(i % S);
     const float g = scratch[i], h = scratch[total + i];
     const float gp = scratch[2 * total + i], hp = scratch[3 * total + i];
     const float shift = (1.0f - prof[s]) * (float)gbar[s];
     const float d = -(g - shift) * h;
     delta[i] = d;
-    pl = (double)(gp * d);
-    pq = (double)((hp * d) * d);
+    pl += (double)(gp * d);
+    pq += (double)((hp * d) * d);
   }
   block_sum2_f64(pl, pq);
   if (threadIdx.x == 0) {
@@ -150,9 +161,13 @@ __global__ __launch_bounds__(kThreads) void vcd_dc_delta_kernel(
   }
 }
 
-__global__ __launch_bounds__(kThreads) void vcd_alpha_kernel(
-    const double* __restrict__ pp, int npp, const double* __restrict__ fp, int nfp,
-    double fm, double max_alpha, float* __restrict__ alpha, double* __restrict__ alpha_sum) {
+// The step size from the prior partials pp (npp pairs) and the line-search
+// partials fp (nfp pairs), FP64 in a fixed order; every thread of the
+// workgroup returns the same value (tomography_model.py:2620-2632).
+__device__ float vcd_alpha_value(const double* __restrict__ pp, int npp,
+                                 const double* __restrict__ fp, int nfp, double fm,
+                                 double max_alpha) {
+  __shared__ float s_alpha;
   double pl = 0.0, pq = 0.0, lin = 0.0, quad = 0.0;
   for (int i = threadIdx.x; i < npp; i += kThreads) { pl += pp[2 * i]; pq += pp[2 * i + 1]; }
   for (int i = threadIdx.x; i < nfp; i += kThreads) { lin += fp[2 * i]; quad += fp[2 * i + 1]; }
@@ -163,7 +178,17 @@ __global__ __launch_bounds__(kThreads) void vcd_alpha_kernel(
     double a = (fm * lin - pl) / (fm * quad + pq + eps);
     if (!(a >= eps)) a = eps;  // also maps NaN to eps, like a clamp of a NaN-free value
     if (a > max_alpha) a = max_alpha;
-    const float af = (float)a;
+    s_alpha = (float)a;
+  }
+  __syncthreads();
+  return s_alpha;
+}
+
+__global__ __launch_bounds__(kThreads) void vcd_alpha_kernel(
+    const double* __restrict__ pp, int npp, const double* __restrict__ fp, int nfp,
+    double fm, double max_alpha, float* __restrict__ alpha, double* __restrict__ alpha_sum) {
+  const float af = vcd_alpha_value(pp, npp, fp, nfp, fm, max_alpha);
+  if (threadIdx.x == 0) {
     alpha[0] = af;
     alpha_sum[0] += (double)af;
   }
@@ -182,18 +207,20 @@ __global__ void vcd_positivity_kernel(const float* __restrict__ x, const int64_t
 }
 
 // x[pix[p]*S + s] += alpha*delta[p, s], fused with the per-workgroup stats
-// partials part[b*(S+1) + s] = sum_p (alpha*delta[p, s])^2 and
-// part[b*(S+1) + S] = sum |alpha*delta| over the workgroup's pixel range.
+// part[b*(S+1) + s] = sum_p (alpha*delta[p, s])^2 and
+// part[b*(S+1) + S] = sum |alpha*delta| over the workgroup's pixel range
+// (ACC: added to what part holds instead of stored).
 // Lanes map to slices (coalesced cylinders), waves to pixels.
-__global__ __launch_bounds__(kThreads) void vcd_apply_kernel(
-    float* __restrict__ x, const int64_t* __restrict__ pix, int P, int S,
-    const float* __restrict__ delta, const float* __restrict__ alpha, int ppb,
-    double* __restrict__ part) {
+template <bool ACC>
+__device__ __forceinline__ void vcd_apply_block(float* __restrict__ x,
+                                                const int64_t* __restrict__ pix, int P, int S,
+                                                const float* __restrict__ delta, float a, int ppb,
+                                                int b, double* __restrict__ part) {
   __shared__ double s_red[kThreads / 64][64];
-  const float a = alpha[0];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int NW = kThreads / 64;
-  const int p0 = blockIdx.x * ppb, p1 = min(p0 + ppb, P);
+  const int p0 = b * ppb, p1 = min(p0 + ppb, P);
+  double* row = part + (size_t)b * (S + 1);
   double ell1 = 0.0;
   for (int sc = 0; sc < S; sc += 64) {
     const int s = sc + lane;
@@ -215,23 +242,86 @@ __global__ __launch_bounds__(kThreads) void vcd_apply_kernel(
       double t = 0.0;
 #pragma unroll
       for (int w = 0; w < NW; ++w) t += s_red[w][lane];
-      part[(size_t)blockIdx.x * (S + 1) + s] = t;
+      row[s] = ACC ? row[s] + t : t;
     }
     __syncthreads();
   }
   double unused = 0.0;
   block_sum2_f64(ell1, unused);
-  if (threadIdx.x == 0) part[(size_t)blockIdx.x * (S + 1) + S] = ell1;
+  if (threadIdx.x == 0) row[S] = ACC ? row[S] + ell1 : ell1;
 }
 
-// acc[c] += sum_b part[b*(S+1) + c], one workgroup per column c, fixed order.
+__global__ __launch_bounds__(kThreads) void vcd_apply_kernel(
+    float* __restrict__ x, const int64_t* __restrict__ pix, int P, int S,
+    const float* __restrict__ delta, const float* __restrict__ alpha, int ppb,
+    double* __restrict__ part) {
+  vcd_apply_block<false>(x, pix, P, S, delta, alpha[0], ppb, blockIdx.x, part);
+}
+
+// acc[c] += sum_b part[b*(S+1) + c], one workgroup per column c, fixed order
+// (CLEAR: the column is zeroed after it is read).
+template <bool CLEAR>
 __global__ __launch_bounds__(kThreads) void vcd_stats_final_kernel(
-    const double* __restrict__ part, int nb, int S, double* __restrict__ acc) {
+    double* __restrict__ part, int nb, int S, double* __restrict__ acc) {
   const int c = blockIdx.x;
   double v = 0.0, unused = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kThreads) v += part[(size_t)b * (S + 1) + c];
+  for (int b = threadIdx.x; b < nb; b += kThreads) {
+    v += part[(size_t)b * (S + 1) + c];
+    if (CLEAR) part[(size_t)b * (S + 1) + c] = 0.0;
+  }
   block_sum2_f64(v, unused);
   if (threadIdx.x == 0) acc[c] += v;
+}
+
+// The end of a subset step in one launch.  With dir_parts set, every
+// workgroup first forms the step size from the prior and line-search
+// partials (the same value everywhere; workgroup 0 records it in alpha and
+// alpha_sum); otherwise alpha[0] holds it.  Workgroups [0, nb_update) then
+// stream the error update e -= alpha*A(delta) (the same float32 operations
+// as the projector's update epilogue); the others apply x[pix] +=
+// alpha*delta and add their statistics into the persistent per-workgroup
+// rows of part, which mbir_vcd_block_stats_reduce folds into the
+// iteration's statistics.
+struct FinishArgs {
+  void* e;
+  const float* adelta;
+  int64_t n_sino;
+  int nb_update;
+  float* x;
+  const int64_t* pix;
+  int P, S, ppb;
+  const float* delta;
+  float* alpha;
+  double* alpha_sum;
+  const double* dir_parts;
+  int n_dir;
+  const double* fwd_parts;
+  int n_fwd;
+  double fm, max_alpha;
+  double* part;
+};
+
+template <typename TS>
+__global__ __launch_bounds__(kThreads) void vcd_finish_kernel(FinishArgs a) {
+  float al;
+  if (a.dir_parts) {
+    al = vcd_alpha_value(a.dir_parts, a.n_dir, a.fwd_parts, a.n_fwd, a.fm, a.max_alpha);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.alpha[0] = al;
+      a.alpha_sum[0] += (double)al;
+    }
+  } else {
+    al = a.alpha[0];
+  }
+  if ((int)blockIdx.x < a.nb_update) {
+    TS* e = static_cast<TS*>(a.e);
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < a.n_sino;
+         i += (int64_t)a.nb_update * kThreads)
+      st_s(e + i, ld_s(e + i) - al * a.adelta[i]);
+    return;  // the whole workgroup leaves: no barrier is skipped
+  }
+  vcd_apply_block<true>(a.x, a.pix, a.P, a.S, a.delta, al, a.ppb, blockIdx.x - a.nb_update,
+                        a.part);
 }
 
 static inline int apply_blocks(int P) {
@@ -341,8 +431,9 @@ using namespace mbir;
 extern "C" {
 
 int32_t mbir_vcd_direction_partials_count(int32_t P, int32_t S) {
-  const int64_t total = (int64_t)P * S;
-  return (int32_t)((total + kThreads - 1) / kThreads);
+  // One partial pair per workgroup; large subsets stride over at most 1024.
+  const int64_t nb = ((int64_t)P * S + kThreads - 1) / kThreads;
+  return (int32_t)(nb > 1024 ? 1024 : nb);
 }
 
 int mbir_vcd_direction(const float* x, const int64_t* pix, int32_t P, const float* back_parts,
@@ -417,6 +508,96 @@ int64_t mbir_vcd_apply_scratch(int32_t P, int32_t S) {
   return (int64_t)apply_blocks(P) * (S + 1);
 }
 
+// Partial pairs times workgroups above which the finishing launch stops
+// forming alpha in every workgroup (16 MiB of re-read partials).
+constexpr int64_t kFoldAlphaPairs = 1 << 20;
+
+static int finish_update_blocks(int64_t n) {
+  const int64_t nb = (n + 8LL * kThreads - 1) / (8LL * kThreads);  // >= 8 elements per lane
+  return (int)(nb > 512 ? 512 : nb);
+}
+
+int32_t mbir_vcd_block_stats_rows(int32_t P) { return P > 0 ? apply_blocks(P) : 0; }
+
+static int vcd_finish_any(void* e, const float* adelta, int64_t n_sino, float* x,
+                          const int64_t* pix, int32_t P, int32_t S, const float* delta,
+                          const double* dir_parts, int32_t n_dir, const double* fwd_parts,
+                          int32_t n_fwd, const mbir_vcd_params* prm, float* alpha,
+                          double* alpha_sum, double* block_stats, mbir_stream_t stream,
+                          bool bf16) {
+  if (!x || !pix || !delta || !alpha || !block_stats || n_sino < 0) return MBIR_ERR_ARG;
+  if (n_sino > 0 && (!e || !adelta)) return MBIR_ERR_ARG;
+  if (dir_parts && (!fwd_parts || !prm || !alpha_sum || n_dir < 0 || n_fwd < 0))
+    return MBIR_ERR_ARG;
+  if (P <= 0 || S <= 0) return MBIR_ERR_SHAPE;
+  FinishArgs a;
+  a.e = e;
+  a.adelta = adelta;
+  a.n_sino = n_sino;
+  a.nb_update = finish_update_blocks(n_sino);
+  a.x = x;
+  a.pix = pix;
+  a.P = P;
+  a.S = S;
+  const int nb_a = apply_blocks(P);
+  a.ppb = (P + nb_a - 1) / nb_a;
+  a.delta = delta;
+  a.alpha = alpha;
+  a.alpha_sum = alpha_sum;
+  a.dir_parts = dir_parts;
+  a.n_dir = n_dir;
+  a.fwd_parts = fwd_parts;
+  a.n_fwd = n_fwd;
+  a.fm = prm ? (double)prm->fm_constant : 0.0;
+  a.max_alpha = prm ? (double)prm->max_alpha : 0.0;
+  a.part = block_stats;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t grid = (int64_t)a.nb_update + nb_a;
+  if (dir_parts && (int64_t)(n_dir + n_fwd) * grid > kFoldAlphaPairs) {
+    // Every workgroup would re-read too many partials: one workgroup forms
+    // alpha first (the same arithmetic), the finishing launch reads it.
+    vcd_alpha_kernel<<<1, kThreads, 0, s>>>(dir_parts, n_dir, fwd_parts, n_fwd, a.fm,
+                                            a.max_alpha, alpha, alpha_sum);
+    MBIR_CHECK_LAUNCH();
+    a.dir_parts = nullptr;
+  }
+  if (bf16)
+    vcd_finish_kernel<bf16_bits><<<a.nb_update + nb_a, kThreads, 0, s>>>(a);
+  else
+    vcd_finish_kernel<float><<<a.nb_update + nb_a, kThreads, 0, s>>>(a);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
+int mbir_vcd_finish(float* error_vcr, const float* adelta_vcr, int64_t n_sino, float* x,
+                    const int64_t* pix, int32_t P, int32_t S, const float* delta,
+                    const double* dir_parts, int32_t n_dir, const double* fwd_parts,
+                    int32_t n_fwd, const mbir_vcd_params* prm, float* alpha, double* alpha_sum,
+                    double* block_stats, mbir_stream_t stream) {
+  return vcd_finish_any(error_vcr, adelta_vcr, n_sino, x, pix, P, S, delta, dir_parts, n_dir,
+                        fwd_parts, n_fwd, prm, alpha, alpha_sum, block_stats, stream, false);
+}
+
+int mbir_vcd_finish_bf16(uint16_t* error_vcr, const float* adelta_vcr, int64_t n_sino, float* x,
+                         const int64_t* pix, int32_t P, int32_t S, const float* delta,
+                         const double* dir_parts, int32_t n_dir, const double* fwd_parts,
+                         int32_t n_fwd, const mbir_vcd_params* prm, float* alpha,
+                         double* alpha_sum, double* block_stats, mbir_stream_t stream) {
+  return vcd_finish_any(error_vcr, adelta_vcr, n_sino, x, pix, P, S, delta, dir_parts, n_dir,
+                        fwd_parts, n_fwd, prm, alpha, alpha_sum, block_stats, stream, true);
+}
+
+int mbir_vcd_block_stats_reduce(double* block_stats, int32_t rows, int32_t S, double* acc,
+                                mbir_stream_t stream) {
+  if (!block_stats || !acc || rows < 0) return MBIR_ERR_ARG;
+  if (S <= 0) return MBIR_ERR_SHAPE;
+  if (rows == 0) return MBIR_OK;
+  vcd_stats_final_kernel<true><<<S + 1, kThreads, 0, (hipStream_t)stream>>>(block_stats, rows,
+                                                                             S, acc);
+  MBIR_CHECK_LAUNCH();
+  return MBIR_OK;
+}
+
 int mbir_vcd_apply(float* x, const int64_t* pix, int32_t P, int32_t S, const float* delta,
                    const float* alpha, double* acc, double* scratch, mbir_stream_t stream) {
   if (!x || !pix || !delta || !alpha || !acc || !scratch) return MBIR_ERR_ARG;
@@ -426,7 +607,7 @@ int mbir_vcd_apply(float* x, const int64_t* pix, int32_t P, int32_t S, const flo
   const int ppb = (P + nb - 1) / nb;
   vcd_apply_kernel<<<nb, kThreads, 0, s>>>(x, pix, P, S, delta, alpha, ppb, scratch);
   MBIR_CHECK_LAUNCH();
-  vcd_stats_final_kernel<<<S + 1, kThreads, 0, s>>>(scratch, nb, S, acc);
+  vcd_stats_final_kernel<false><<<S + 1, kThreads, 0, s>>>(scratch, nb, S, acc);
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }

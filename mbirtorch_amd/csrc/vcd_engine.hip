@@ -11,6 +11,8 @@
 // permutation, drawn on the host).
 #include "common.h"
 
+#include <algorithm>
+
 using namespace mbir;
 
 namespace {
@@ -91,17 +93,23 @@ int subset_step(const mbir_vcd_job* j, int k, double* stats, hipStream_t s) {
                                         j->error_vcr, j->weights_vcr, adelta, j->fwd_parts, s));
     n_fwd = mbir_cone_fwd_partials_count(j->cone_geom);
   }
+  // The positivity clip changes delta after the line search, so A(delta)
+  // is re-projected then; otherwise the stored A(delta) is streamed by the
+  // finishing launch, which also forms alpha.
+  double* block_stats = j->apply_scratch;
+  if (j->adelta && !j->positivity) {
+    if (h)
+      return mbir_vcd_finish_bf16(e16w, j->adelta, j->sino_size, j->flat_recon, pix, P, S,
+                                  j->delta, j->dir_parts, n_dir, j->fwd_parts, n_fwd, &j->prm,
+                                  j->alpha, stats + S + 1, block_stats, s);
+    return mbir_vcd_finish(j->error_vcr, j->adelta, j->sino_size, j->flat_recon, pix, P, S,
+                           j->delta, j->dir_parts, n_dir, j->fwd_parts, n_fwd, &j->prm, j->alpha,
+                           stats + S + 1, block_stats, s);
+  }
   MBIR_TRY(mbir_vcd_alpha(j->dir_parts, n_dir, j->fwd_parts, n_fwd, &j->prm, j->alpha,
                           stats + S + 1, s));
   if (j->positivity) MBIR_TRY(mbir_vcd_positivity(j->flat_recon, pix, P, S, j->alpha, j->delta, s));
-  // The positivity clip changes delta after the line search, so A(delta)
-  // is re-projected then; otherwise the stored A(delta) is streamed.
-  const bool stored = j->adelta && !j->positivity;
-  if (stored && h)
-    MBIR_TRY(mbir_vcd_error_update_bf16(e16w, j->adelta, j->alpha, j->sino_size, s));
-  else if (stored)
-    MBIR_TRY(mbir_vcd_error_update(j->error_vcr, j->adelta, j->alpha, j->sino_size, s));
-  else if (par && h)
+  if (par && h)
     MBIR_TRY(mbir_par_fwd_update_bf16(j->delta, pix, P, j->view_table, j->par_geom, j->alpha,
                                       e16w, s));
   else if (par)
@@ -113,7 +121,8 @@ int subset_step(const mbir_vcd_job* j, int k, double* stats, hipStream_t s) {
   else
     MBIR_TRY(mbir_cone_fwd_update(j->delta, pix, P, j->view_table, j->cone_geom, j->alpha,
                                   j->error_vcr, s));
-  return mbir_vcd_apply(j->flat_recon, pix, P, S, j->delta, j->alpha, stats, j->apply_scratch, s);
+  return mbir_vcd_finish(nullptr, nullptr, 0, j->flat_recon, pix, P, S, j->delta, nullptr, 0,
+                         nullptr, 0, nullptr, j->alpha, nullptr, block_stats, s);
 }
 
 }  // namespace
@@ -132,6 +141,10 @@ int mbir_vcd_run(const mbir_vcd_job* job, const int32_t* subsets, int32_t num_st
   }
   if (end_iteration) {
     const int S = job->prm.num_slices;
+    int rows = 0;
+    for (int k = 0; k < job->num_subsets; ++k)
+      rows = std::max(rows, (int)mbir_vcd_block_stats_rows(job->subset_sizes[k]));
+    MBIR_TRY(mbir_vcd_block_stats_reduce(job->apply_scratch, rows, S, stats, s));
     if (job->sino_dtype == MBIR_DTYPE_BF16)
       MBIR_TRY(mbir_sumsq_bf16(reinterpret_cast<const uint16_t*>(job->error_vcr),
                                reinterpret_cast<const uint16_t*>(job->weights_vcr),

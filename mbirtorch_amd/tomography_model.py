@@ -763,8 +763,10 @@ class _VcdEngine:
         n_fwd = max(ops.fwd_partials_count(ray=False), ops.fwd_partials_count(ray=True))
         self.fwd_parts = torch.empty(2 * n_fwd, dtype=torch.float64, device=dev)
         self.alpha = torch.zeros(1, dtype=torch.float32, device=dev)
+        # Per-workgroup statistics rows, carried to each iteration's end
+        # (which clears them): zero before the first step.
         n_apply = int(lib.mbir_vcd_apply_scratch(max_p, S))
-        self.apply_scratch = torch.empty(max(n_apply, 1), dtype=torch.float64, device=dev)
+        self.apply_scratch = torch.zeros(max(n_apply, 1), dtype=torch.float64, device=dev)
         # A(delta) of the current subset, kept by the line search so the error
         # update is one streaming pass (sinogram-sized float32).
         self.adelta = torch.empty(e.numel(), dtype=torch.float32, device=dev)
