@@ -27,9 +27,11 @@
 //    in), evaluates the 3-5 candidate pixels of the channel exactly, and
 //    accumulates in registers;
 //  * binned: for arbitrary pixel lists (the VCD subsets), a workgroup owns one
-//    view and 256 channels; chunks of the list are counting-sorted by channel
-//    in LDS, each bin ordered by list position, and every thread gathers its
-//    channel's bin.  Its epilogue writes the projection, or -- for the VCD
+//    view and a block of channels; chunks of the list are counting-sorted in
+//    LDS by first tap channel, each bin ordered by list position, and each
+//    unit walks the contiguous run of sorted pixels whose taps reach its
+//    channels, as the ray form walks image rows.  Its epilogue writes the
+//    projection, or -- for the VCD
 //    subset step -- reduces it against the error sinogram (line search, FP64)
 //    or subtracts it from the error sinogram, so the subset direction's
 //    projection never goes to HBM.
@@ -255,8 +257,10 @@ __device__ __forceinline__ void load_vr(const float* p, float (&x)[VR]) {
 
 template <int VR>
 __device__ __forceinline__ void store_vr(float* p, const float (&x)[VR]) {
-  if constexpr (VR == 4) {
-    *reinterpret_cast<float4*>(p) = make_float4(x[0], x[1], x[2], x[3]);
+  if constexpr (VR % 4 == 0) {
+#pragma unroll
+    for (int h = 0; h < VR; h += 4)
+      *reinterpret_cast<float4*>(p + h) = make_float4(x[h], x[h + 1], x[h + 2], x[h + 3]);
   } else {
 #pragma unroll
     for (int q = 0; q < VR; ++q) p[q] = x[q];
@@ -546,22 +550,20 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Binned forward for arbitrary pixel lists.  Workgroup = (view, 256
-// channels, row chunk).  Each chunk of the list is counting-sorted by channel
-// in LDS (histogram, scan, then a placement whose order depends only on the
-// list: pixel slot, tap, thread), and unit u gathers the bins of channels
-// c0 + u + k*UNITS, k < LPC, accumulating in registers across chunks.  The
-// summation order of every sinogram entry is fixed by the list:
-// deterministic.
-constexpr int kShortBin = 24;  // longest bin the insertion sort takes
-
-template <bool TWO>
-struct BinCfg {
-  static constexpr int NT = TWO ? 2 : kMaxTaps;
-  static constexpr int CH = TWO ? 1024 : 256;  // pixels per chunk
-  static constexpr int PPT = CH / kBlock;      // pixels per thread
-  static constexpr int ENT = CH * NT;          // entry capacity
-};
+// Binned forward for arbitrary pixel lists (the VCD subsets).  Workgroup =
+// (view, CHW channels, row chunk); units own CPS adjacent channels as in the
+// ray form.  Each chunk of the list is counting-sorted in LDS by the first
+// tap channel of its pixels -- one entry per pixel: list position, bin and
+// the two tap weights (two-tap rule) or n_p (2*psf+1 taps) -- bins ordered by
+// list position.  The pixels whose taps reach a unit's channels n0 ..
+// n0+CPS-1 are then one contiguous segment of the sorted chunk (first taps
+// n0-ntaps+1 .. n0+CPS-1), which the unit walks NB candidates at a time:
+// tap weights for its CPS channels, the candidates' cylinder loads in flight
+// together, then the FMAs.  A pixel's cylinder is loaded by the one or two
+// units its taps reach.  The summation order of every sinogram entry is
+// fixed by the list: deterministic.
+constexpr int kSegChunk = 2048;   // pixels per chunk (positions fit 16 bits)
+constexpr int kSegShortBin = 32;  // longest bin ranked by counting
 
 // Exclusive block scan of one int per thread (256 threads).
 __device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp) {
@@ -579,161 +581,207 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp) {
   return prefix + x - v;
 }
 
-template <int LPC, int VR, bool TWO, int MODE, bool WEIGHTED, typename TS>
+template <int LPC, int VR, int CPS, int NB, bool TWO, int MODE, bool WEIGHTED, typename TS>
 __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
     const float* __restrict__ values, const int64_t* __restrict__ pix, int n_pix,
     const ParView* __restrict__ views, mbir_par_geom g, TS* __restrict__ sino,
     const TS* __restrict__ wts, const float* __restrict__ alpha,
     double* __restrict__ partials, float* __restrict__ adelta) {
   using L = FwdLanes<LPC, VR>;
-  using Cfg = BinCfg<TWO>;
-  constexpr int NT = Cfg::NT;
-  __shared__ int s_cnt[kBlock + 1];
-  __shared__ int s_off[kBlock + 1];
+  constexpr int CHW = L::UNITS * CPS;
+  constexpr int NBIN_MAX = CHW + (TWO ? 1 : kMaxTaps - 1);
+  constexpr int PPT = kSegChunk / kBlock;
+  constexpr int BPT = (NBIN_MAX + kBlock - 1) / kBlock;  // bins per thread in the scan
+  __shared__ int s_cnt[BPT * kBlock];
+  __shared__ int s_off[BPT * kBlock + 1];
   __shared__ int s_tmp[4];
-  __shared__ int s_ej[Cfg::ENT];
-  __shared__ float s_ea[Cfg::ENT];
-  __shared__ int s_wcnt[kBlock / MBIR_WAVE][kBlock];
+  __shared__ int s_pk[kSegChunk];  // list position | bin << 16
+  __shared__ int s_wa[kSegChunk];  // float bits: first tap weight (two-tap) or n_p
+  __shared__ int s_wb[TWO ? kSegChunk : 1];  // float bits: second tap weight
+  // Unsorted list positions of the short-bin placement, in the weight array
+  // that the sorted entries fill only after every rank is known.
+  int* const s_pos2 = TWO ? s_wb : s_wa;
+  __shared__ int s_wcnt[kBlock / MBIR_WAVE][NBIN_MAX];
 
   const int tid = threadIdx.x;
   const int lane = tid & (MBIR_WAVE - 1), wave = tid / MBIR_WAVE;
   const int unit = wave * L::SPW + lane / LPC, rl = lane % LPC;
-  const int v = blockIdx.x, c0 = blockIdx.y * kBlock, r0 = blockIdx.z * L::ROWS;
-#pragma unroll
-  for (int w = 0; w < kBlock / MBIR_WAVE; ++w) s_wcnt[w][tid] = 0;
-  const int R = g.num_slices, C = g.num_channels;
+  const int v = blockIdx.x, c0 = blockIdx.y * CHW, r0 = blockIdx.z * L::ROWS;
+  const int R = g.num_slices, C = g.num_channels, Nc = g.num_cols;
   const int rr = r0 + rl * VR;
   const bool row_ok = rr < R;
   const int ntaps = TWO ? 2 : 2 * g.psf_radius + 1;
+  const int nbin = CHW + ntaps - 1;
+  const int lo = c0 - (ntaps - 1);  // first-tap channel of bin 0
+  const int n0 = c0 + unit * CPS;
+  const int tb0 = unit * CPS + ntaps - 1;  // bin whose first tap is n0
   const ParView vw = views[v];
+  for (int i = tid; i < (kBlock / MBIR_WAVE) * NBIN_MAX; i += kBlock) (&s_wcnt[0][0])[i] = 0;
 
-  float acc[LPC][VR];
+  float acc[CPS][VR];
 #pragma unroll
-  for (int k = 0; k < LPC; ++k)
+  for (int k = 0; k < CPS; ++k)
 #pragma unroll
     for (int q = 0; q < VR; ++q) acc[k][q] = 0.0f;
 
-  for (int pc = 0; pc < n_pix; pc += Cfg::CH) {
-    // Geometry of this thread's pixels; taps landing in [c0, c0+256) count.
-    int bin[Cfg::PPT][NT];
-    float wgt[Cfg::PPT][NT];
-    s_cnt[tid] = 0;
+  for (int pc = 0; pc < n_pix; pc += kSegChunk) {
+    int bin[PPT];
+    float npv[PPT];
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) s_cnt[tid * BPT + i] = 0;
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < Cfg::PPT; ++q) {
+    for (int q = 0; q < PPT; ++q) {
       const int j = pc + tid + q * kBlock;
-      int base = 0;
-      float a[NT];
+      int b = -1;
+      float n_p = 0.0f;
       if (j < n_pix) {
-        fwd_taps<NT, TWO>(pix[j], vw, g, ntaps, base, a);
-      } else {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) a[t] = 0.0f;
+        const int pj = (int)pix[j];  // pixel indices < rows*cols < 2^31
+        const int row = pj / Nc, col = pj - row * Nc;
+        n_p = par_np(row, col, vw, g);
+        b = par_base<TWO ? 2 : 0>(n_p, g) - lo;
+        if (b < 0 || b >= nbin) b = -1;
       }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int b = base + t - c0;
-        const bool in = a[t] != 0.0f && b >= 0 && b < kBlock;
-        bin[q][t] = in ? b : -1;
-        wgt[q][t] = a[t];
-        if (in) atomicAdd(&s_cnt[b], 1);
-      }
+      bin[q] = b;
+      npv[q] = n_p;
+      if (b >= 0) atomicAdd(&s_cnt[b], 1);
     }
     __syncthreads();
-    const int mine = s_cnt[tid];
-    const int off = block_exclusive_scan(mine, s_tmp);
-    s_off[tid] = off;
-    if (tid == kBlock - 1) s_off[kBlock] = off + mine;
-    s_cnt[tid] = off;  // placement cursor of bin tid
-    // Two deterministic placements.  Short bins (random subsets): scatter with
-    // LDS atomics, then an insertion sort of each bin by list position.  Long
-    // bins (an image row of a dense list projecting onto one channel), where
-    // that sort is quadratic: place round by round (pixel slot q, tap t) and
-    // within a round by thread id -- the bin cursor, plus the same-bin
-    // entries of lower waves (s_wcnt), plus the rank among same-bin lanes
-    // below in the wave.
-    if (!__syncthreads_or(mine > kShortBin)) {
+    // Bins tid*BPT .. tid*BPT+BPT-1 (counts zero beyond nbin).
+    int cnt[BPT], mine = 0, longest = 0;
 #pragma unroll
-      for (int q = 0; q < Cfg::PPT; ++q) {
+    for (int i = 0; i < BPT; ++i) {
+      cnt[i] = s_cnt[tid * BPT + i];
+      mine += cnt[i];
+      longest = max(longest, cnt[i]);
+    }
+    int off = block_exclusive_scan(mine, s_tmp);
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const int b = bin[q][t];
-          if (b >= 0) {
-            const int slot = atomicAdd(&s_cnt[b], 1);
-            s_ej[slot] = tid + q * kBlock;  // position within the chunk
-            s_ea[slot] = wgt[q][t];
-          }
-        }
+    for (int i = 0; i < BPT; ++i) {
+      s_off[tid * BPT + i] = off;
+      s_cnt[tid * BPT + i] = off;  // placement cursor of the bin
+      off += cnt[i];
+    }
+    if (tid == kBlock - 1) s_off[BPT * kBlock] = off;
+    // Entry of slot q at its sorted place sl.
+    auto put = [&](int q, int sl) {
+      const int me = tid + q * kBlock;
+      s_pk[sl] = me | (bin[q] << 16);
+      if constexpr (TWO) {
+        const int base = lo + bin[q];
+        s_wa[sl] = __float_as_int(par_weight(npv[q], base, vw));
+        s_wb[sl] = __float_as_int(par_weight(npv[q], base + 1, vw));
+      } else {
+        s_wa[sl] = __float_as_int(npv[q]);
       }
+    };
+    if (!__syncthreads_or(longest > kSegShortBin)) {
+      // Short bins: place with LDS atomics, then move each entry to its rank
+      // by list position within the bin (positions in a chunk are distinct).
+#pragma unroll
+      for (int q = 0; q < PPT; ++q)
+        if (bin[q] >= 0) s_pos2[atomicAdd(&s_cnt[bin[q]], 1)] = tid + q * kBlock;
       __syncthreads();
-      const int e0 = s_off[tid], e1 = s_off[tid + 1];
-      for (int e = e0 + 1; e < e1; ++e) {
-        const int kj = s_ej[e];
-        const float ka = s_ea[e];
-        int f = e - 1;
-        while (f >= e0 && s_ej[f] > kj) {
-          s_ej[f + 1] = s_ej[f];
-          s_ea[f + 1] = s_ea[f];
-          --f;
+      int slot[PPT];
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        slot[q] = -1;
+        if (bin[q] >= 0) {
+          const int e0 = s_off[bin[q]], e1 = s_off[bin[q] + 1];
+          const int me = tid + q * kBlock;
+          int rank = 0;
+          for (int e = e0; e < e1; ++e) rank += s_pos2[e] < me;
+          slot[q] = e0 + rank;
         }
-        s_ej[f + 1] = kj;
-        s_ea[f + 1] = ka;
       }
+      __syncthreads();  // every rank read before the weights overwrite s_pos2
+#pragma unroll
+      for (int q = 0; q < PPT; ++q)
+        if (slot[q] >= 0) put(q, slot[q]);
     } else {
+      // Long bins (a dense list: an image row onto one channel): stable
+      // placement round by round (pixel slot q) and within a round by thread
+      // id -- the bin cursor, plus the same-bin entries of lower waves, plus
+      // the rank among same-bin lanes below in the wave.
 #pragma unroll
-      for (int q = 0; q < Cfg::PPT; ++q) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const int b = bin[q][t];
-          int rank, gsize;
-          bool leader;
-          wave_bin_rank(b, rank, gsize, leader);
-          if (leader) s_wcnt[wave][b] = gsize;
-          if (!__syncthreads_or(b >= 0)) continue;
-          if (b >= 0) {
-            int slot = s_cnt[b] + rank;
-            for (int w = 0; w < wave; ++w) slot += s_wcnt[w][b];
-            s_ej[slot] = tid + q * kBlock;
-            s_ea[slot] = wgt[q][t];
-          }
-          __syncthreads();
+      for (int q = 0; q < PPT; ++q) {
+        const int b = bin[q];
+        int rank, gsize;
+        bool leader;
+        wave_bin_rank(b, rank, gsize, leader);
+        if (leader) s_wcnt[wave][b] = gsize;
+        __syncthreads();
+        if (b >= 0) {
+          int sl = s_cnt[b] + rank;
+          for (int w = 0; w < wave; ++w) sl += s_wcnt[w][b];
+          put(q, sl);
+        }
+        __syncthreads();
+        for (int bb = tid; bb < nbin; bb += kBlock) {
           int add = 0;
 #pragma unroll
           for (int w = 0; w < kBlock / MBIR_WAVE; ++w) {
-            add += s_wcnt[w][tid];
-            s_wcnt[w][tid] = 0;
+            add += s_wcnt[w][bb];
+            s_wcnt[w][bb] = 0;
           }
-          s_cnt[tid] += add;
-          __syncthreads();
+          s_cnt[bb] += add;
         }
+        __syncthreads();
       }
     }
     __syncthreads();
     if (row_ok) {
-      // Rounds over the unit's LPC bins: round r takes entry r of every bin,
-      // so up to LPC independent cylinder loads are in flight per lane.
       const float* xb = values + (size_t)pc * R + rr;
-      int span[LPC];  // (bin start << 16) | bin length; ENT < 65536
-      int rounds = 0;
+      const int b0 = unit * CPS;  // first taps n0-ntaps+1 .. n0+CPS-1
+      const int s0 = s_off[b0], s1 = s_off[min(b0 + CPS + ntaps - 1, nbin)];
+      for (int e = s0; e < s1; e += NB) {
+        float a[NB][CPS];
+        int pos[NB];
+        bool use[NB];
 #pragma unroll
-      for (int k = 0; k < LPC; ++k) {
-        const int ch = unit + k * L::UNITS;
-        const int e0 = s_off[ch], len = s_off[ch + 1] - e0;
-        span[k] = (e0 << 16) | len;
-        rounds = max(rounds, len);
-      }
-      for (int r = 0; r < rounds; ++r) {
+        for (int u = 0; u < NB; ++u) {
+          const bool ok = e + u < s1;
+          const int ee = ok ? e + u : s0;
+          const int pk = s_pk[ee];
+          pos[u] = pk & 0xffff;
+          bool any = false;
+          if constexpr (TWO) {
+            // Taps n0+t and n0+t+1 carry the stored weights.
+            const int t = (pk >> 16) - tb0;
+            const float w0 = __int_as_float(s_wa[ee]), w1 = __int_as_float(s_wb[ee]);
 #pragma unroll
-        for (int k = 0; k < LPC; ++k) {
-          if (r < (span[k] & 0xffff)) {
-            const int e = (span[k] >> 16) + r;
-            const float a = s_ea[e];
-            float x[VR];
-            load_vr<VR>(xb + (size_t)s_ej[e] * R, x);
+            for (int k = 0; k < CPS; ++k) {
+              a[u][k] = t == k ? w0 : (t == k - 1 ? w1 : 0.0f);
+              any = any || a[u][k] != 0.0f;
+            }
+          } else {
+            const float n_p = __int_as_float(s_wa[ee]);
+            const int t0 = (pk >> 16) - tb0;  // first tap - n0
 #pragma unroll
-            for (int q = 0; q < VR; ++q) acc[k][q] = fmaf(a, x[q], acc[k][q]);
+            for (int k = 0; k < CPS; ++k) {
+              const int t = k - t0;
+              a[u][k] = (t >= 0 && t < ntaps) ? par_weight(n_p, n0 + k, vw) : 0.0f;
+              any = any || a[u][k] != 0.0f;
+            }
+          }
+          use[u] = ok && any;
+        }
+        float x[NB][VR];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          if (use[u]) {
+            load_vr<VR>(xb + (size_t)pos[u] * R, x[u]);
+          } else {
+#pragma unroll
+            for (int q = 0; q < VR; ++q) x[u][q] = 0.0f;
           }
         }
+#pragma unroll
+        for (int u = 0; u < NB; ++u)
+#pragma unroll
+          for (int k = 0; k < CPS; ++k)
+#pragma unroll
+            for (int q = 0; q < VR; ++q) acc[k][q] = fmaf(a[u][k], x[u][q], acc[k][q]);
       }
     }
     __syncthreads();
@@ -743,13 +791,14 @@ __global__ __launch_bounds__(kBlock) void par_fwd_bin_kernel(
   const float al = MODE == FWD_UPDATE ? alpha[0] : 0.0f;
   if (row_ok) {
 #pragma unroll
-    for (int k = 0; k < LPC; ++k) {
-      const int n = c0 + unit + k * L::UNITS;
+    for (int k = 0; k < CPS; ++k) {
+      const int n = n0 + k;
       if (n >= C) continue;
       if (MODE == FWD_WRITE_VRC) {
         if constexpr (sizeof(TS) == sizeof(float)) {
 #pragma unroll
-          for (int q = 0; q < VR; ++q) sino[((size_t)v * R + rr + q) * C + n] = acc[k][q];
+          for (int q = 0; q < VR; ++q)
+            if (rr + q < R) sino[((size_t)v * R + rr + q) * C + n] = acc[k][q];
         }
       } else {
         fwd_epilogue_vcr<VR, MODE, WEIGHTED>(acc[k], ((size_t)v * C + n) * R + rr, sino, wts,
@@ -845,26 +894,39 @@ struct FwdArgs {
   float* adelta;  // line search: also store A(delta) (channel-major), or NULL
 };
 
-// Lane configurations (lanes per unit, rows per lane).  VR = 4 or 8 rows per
-// lane (float4 loads) exactly when R % 4 == 0; the row arrays must then be
-// 16-byte aligned.
+// Lane configurations: lanes per unit, rows per lane (VR = 4, 8 or 16:
+// float4 loads, exactly when R % 4 == 0; the row arrays must then be 16-byte
+// aligned), channels per unit.
 struct LaneCfg {
-  int lpc, vr;
+  int lpc, vr, cps;
 };
 static inline bool fwd_vec(const mbir_par_geom& g) { return g.num_slices % 4 == 0; }
-static inline LaneCfg bin_lanes(int R, bool vec) {
-  if (vec) return {R <= 8 ? 2 : (R <= 16 ? 4 : (R <= 32 ? 8 : 16)), 4};
-  return {R <= 16 ? 16 : 64, 1};
-}
 // The ray form repeats a candidate's geometry on every lane of its unit, so
 // tall cylinders take eight rows per lane (fewer lanes per unit).  (Thinner
 // units -- two lanes of 32 rows, one channel -- issue fewer VALU
 // instructions but ~1.8x the cylinder loads, and measured 1.7x slower.)
 static inline LaneCfg ray_lanes(int R, bool vec) {
-  if (vec && R > 32 && R % 8 == 0) return {8, 8};
-  return bin_lanes(R, vec);
+  if (vec && R > 32 && R % 8 == 0) return {8, 8, 4};
+  if (vec) {
+    const int lpc = R <= 8 ? 2 : (R <= 16 ? 4 : (R <= 32 ? 8 : 16));
+    return {lpc, 4, lpc >= 8 ? 4 : (lpc == 4 ? 2 : 1)};
+  }
+  return {R <= 16 ? 16 : 64, 1, 4};
 }
-static inline int ray_cps(int lpc) { return lpc >= 8 ? 4 : (lpc == 4 ? 2 : 1); }
+// The binned form reads a candidate's tap weights from LDS and loads its
+// cylinder once per unit, so its per-candidate work shrinks with fewer lanes
+// per unit.  Measured at 400 views x 512 channels, 1602- and 12814-pixel
+// subsets: 32 rows as 4 lanes x 8 rows x 2 channels beats 8 x 4 x 4 by
+// 1.05-1.55x; 64 rows keep 8 x 8 x 4.
+static inline LaneCfg bin_lanes(int R, bool vec) {
+  if (vec) {
+    if (R <= 8) return {2, 4, 2};
+    if (R <= 16) return {4, 4, 2};
+    if (R <= 32) return R % 8 == 0 ? LaneCfg{4, 8, 2} : LaneCfg{8, 4, 4};
+    return R % 8 == 0 ? LaneCfg{8, 8, 4} : LaneCfg{16, 4, 4};
+  }
+  return {R <= 16 ? 16 : 64, 1, 4};
+}
 
 // Workgroup count of the ray (ray = true) or binned form: the number of
 // line-search partial pairs it writes.
@@ -873,80 +935,100 @@ static long fwd_blocks(const mbir_par_geom& g, bool ray) {
   const LaneCfg c = ray ? ray_lanes(g.num_slices, vec) : bin_lanes(g.num_slices, vec);
   const long rows = (long)c.lpc * c.vr;
   const long z = (g.num_slices + rows - 1) / rows;
-  const long cb = ray ? (kBlock / c.lpc) * ray_cps(c.lpc) : kBlock;
+  const long cb = (long)(kBlock / c.lpc) * c.cps;
   return (long)g.num_views * ((g.num_channels + cb - 1) / cb) * z;
 }
 
-template <int LPC, int VR, bool TWO, int MODE, bool WEIGHTED>
-static void launch_fwd_kernel(const FwdArgs& a) {
+template <int LPC, int VR, int CPS, bool TWO, int MODE, bool WEIGHTED>
+static void launch_ray(const FwdArgs& a) {
   const mbir_par_geom& g = a.g;
-  constexpr int ROWS = LPC * VR;
-  const int z = (g.num_slices + ROWS - 1) / ROWS;
-  if (a.runs) {
-    constexpr int CPS = LPC >= 8 ? 4 : (LPC == 4 ? 2 : 1);
-    constexpr int NB = 4;  // candidates per batch
-    constexpr int CHW = (kBlock / LPC) * CPS;
-    dim3 grid((g.num_channels + CHW - 1) / CHW, g.num_views, z);
-    par_fwd_ray_kernel<LPC, VR, CPS, NB, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
-        a.values, a.runs, a.views, g, static_cast<float*>(a.sino),
-        static_cast<const float*>(a.w), a.alpha, a.partials, a.adelta);
-  } else {
-    dim3 grid(g.num_views, (g.num_channels + kBlock - 1) / kBlock, z);
-    par_fwd_bin_kernel<LPC, VR, TWO, MODE, WEIGHTED, float><<<grid, kBlock, 0, a.s>>>(
-        a.values, a.pix, a.P, a.views, g, static_cast<float*>(a.sino),
-        static_cast<const float*>(a.w), a.alpha, a.partials, a.adelta);
-  }
+  constexpr int NB = 4;  // candidates per batch
+  constexpr int CHW = (kBlock / LPC) * CPS;
+  dim3 grid((g.num_channels + CHW - 1) / CHW, g.num_views,
+            (g.num_slices + LPC * VR - 1) / (LPC * VR));
+  par_fwd_ray_kernel<LPC, VR, CPS, NB, TWO, MODE, WEIGHTED><<<grid, kBlock, 0, a.s>>>(
+      a.values, a.runs, a.views, g, static_cast<float*>(a.sino),
+      static_cast<const float*>(a.w), a.alpha, a.partials, a.adelta);
 }
 
-template <int LPC, int VR, bool TWO, int MODE, bool WEIGHTED>
-static void launch_fwd_bf16(const FwdArgs& a) {
+template <int LPC, int VR, int CPS, bool TWO, int MODE, bool WEIGHTED, typename TS>
+static void launch_bin(const FwdArgs& a) {
   const mbir_par_geom& g = a.g;
-  constexpr int ROWS = LPC * VR;
-  dim3 grid(g.num_views, (g.num_channels + kBlock - 1) / kBlock,
-            (g.num_slices + ROWS - 1) / ROWS);
-  par_fwd_bin_kernel<LPC, VR, TWO, MODE, WEIGHTED, bf16_bits><<<grid, kBlock, 0, a.s>>>(
-      a.values, a.pix, a.P, a.views, g, static_cast<bf16_bits*>(a.sino),
-      static_cast<const bf16_bits*>(a.w), a.alpha, a.partials, a.adelta);
+  constexpr int NB = 2;  // candidates per batch
+  constexpr int CHW = (kBlock / LPC) * CPS;
+  dim3 grid(g.num_views, (g.num_channels + CHW - 1) / CHW,
+            (g.num_slices + LPC * VR - 1) / (LPC * VR));
+  par_fwd_bin_kernel<LPC, VR, CPS, NB, TWO, MODE, WEIGHTED, TS><<<grid, kBlock, 0, a.s>>>(
+      a.values, a.pix, a.P, a.views, g, static_cast<TS*>(a.sino),
+      static_cast<const TS*>(a.w), a.alpha, a.partials, a.adelta);
 }
 
-template <int LPC, int VR, bool TWO>
+// Mode dispatch for one lane configuration; bfloat16 storage serves the
+// binned line search and update only.
+template <int LPC, int VR, int CPS, bool TWO, bool RAY>
 static int launch_fwd_mode(const FwdArgs& a) {
-  if (a.bf16) {
-    if (a.mode == FWD_LINESEARCH) {
-      if (a.w) launch_fwd_bf16<LPC, VR, TWO, FWD_LINESEARCH, true>(a);
-      else launch_fwd_bf16<LPC, VR, TWO, FWD_LINESEARCH, false>(a);
-    } else {
-      launch_fwd_bf16<LPC, VR, TWO, FWD_UPDATE, false>(a);
+  if constexpr (!RAY) {
+    if (a.bf16) {
+      if (a.mode == FWD_LINESEARCH) {
+        if (a.w) launch_bin<LPC, VR, CPS, TWO, FWD_LINESEARCH, true, bf16_bits>(a);
+        else launch_bin<LPC, VR, CPS, TWO, FWD_LINESEARCH, false, bf16_bits>(a);
+      } else {
+        launch_bin<LPC, VR, CPS, TWO, FWD_UPDATE, false, bf16_bits>(a);
+      }
+      MBIR_CHECK_LAUNCH();
+      return MBIR_OK;
     }
-    MBIR_CHECK_LAUNCH();
-    return MBIR_OK;
   }
+#define MBIR_FWD_LAUNCH(MODE, W)                                \
+  do {                                                          \
+    if constexpr (RAY) launch_ray<LPC, VR, CPS, TWO, MODE, W>(a); \
+    else launch_bin<LPC, VR, CPS, TWO, MODE, W, float>(a);      \
+  } while (0)
   switch (a.mode) {
-    case FWD_WRITE_VCR: launch_fwd_kernel<LPC, VR, TWO, FWD_WRITE_VCR, false>(a); break;
-    case FWD_WRITE_VRC: launch_fwd_kernel<LPC, VR, TWO, FWD_WRITE_VRC, false>(a); break;
+    case FWD_WRITE_VCR: MBIR_FWD_LAUNCH(FWD_WRITE_VCR, false); break;
+    case FWD_WRITE_VRC: MBIR_FWD_LAUNCH(FWD_WRITE_VRC, false); break;
     case FWD_LINESEARCH:
-      if (a.w) launch_fwd_kernel<LPC, VR, TWO, FWD_LINESEARCH, true>(a);
-      else launch_fwd_kernel<LPC, VR, TWO, FWD_LINESEARCH, false>(a);
+      if (a.w) MBIR_FWD_LAUNCH(FWD_LINESEARCH, true);
+      else MBIR_FWD_LAUNCH(FWD_LINESEARCH, false);
       break;
-    default: launch_fwd_kernel<LPC, VR, TWO, FWD_UPDATE, false>(a);
+    default: MBIR_FWD_LAUNCH(FWD_UPDATE, false);
   }
+#undef MBIR_FWD_LAUNCH
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }
 
 template <bool TWO>
 static int launch_fwd_lanes(const FwdArgs& a, bool vec) {
-  const LaneCfg c = a.runs ? ray_lanes(a.g.num_slices, vec) : bin_lanes(a.g.num_slices, vec);
-  if (c.vr == 8) return launch_fwd_mode<8, 8, TWO>(a);
+  if (a.runs) {
+    const LaneCfg c = ray_lanes(a.g.num_slices, vec);
+    if (c.vr == 8) return launch_fwd_mode<8, 8, 4, TWO, true>(a);
+    if (c.vr == 4) {
+      switch (c.lpc) {
+        case 2: return launch_fwd_mode<2, 4, 1, TWO, true>(a);
+        case 4: return launch_fwd_mode<4, 4, 2, TWO, true>(a);
+        case 8: return launch_fwd_mode<8, 4, 4, TWO, true>(a);
+        default: return launch_fwd_mode<16, 4, 4, TWO, true>(a);
+      }
+    }
+    return c.lpc == 16 ? launch_fwd_mode<16, 1, 4, TWO, true>(a)
+                       : launch_fwd_mode<64, 1, 4, TWO, true>(a);
+  }
+  const LaneCfg c = bin_lanes(a.g.num_slices, vec);
+  if (c.vr == 8) {
+    return c.lpc == 4 ? launch_fwd_mode<4, 8, 2, TWO, false>(a)
+                      : launch_fwd_mode<8, 8, 4, TWO, false>(a);
+  }
   if (c.vr == 4) {
     switch (c.lpc) {
-      case 2: return launch_fwd_mode<2, 4, TWO>(a);
-      case 4: return launch_fwd_mode<4, 4, TWO>(a);
-      case 8: return launch_fwd_mode<8, 4, TWO>(a);
-      default: return launch_fwd_mode<16, 4, TWO>(a);
+      case 2: return launch_fwd_mode<2, 4, 2, TWO, false>(a);
+      case 4: return launch_fwd_mode<4, 4, 2, TWO, false>(a);
+      case 8: return launch_fwd_mode<8, 4, 4, TWO, false>(a);
+      default: return launch_fwd_mode<16, 4, 4, TWO, false>(a);
     }
   }
-  return c.lpc == 16 ? launch_fwd_mode<16, 1, TWO>(a) : launch_fwd_mode<64, 1, TWO>(a);
+  return c.lpc == 16 ? launch_fwd_mode<16, 1, 4, TWO, false>(a)
+                     : launch_fwd_mode<64, 1, 4, TWO, false>(a);
 }
 
 static inline bool aligned16(const void* p) {

@@ -509,8 +509,8 @@ int64_t mbir_vcd_apply_scratch(int32_t P, int32_t S) {
 }
 
 // Partial pairs times workgroups above which the finishing launch stops
-// forming alpha in every workgroup (16 MiB of re-read partials).
-constexpr int64_t kFoldAlphaPairs = 1 << 20;
+// forming alpha in every workgroup (64 MiB of re-read partials, from L2).
+constexpr int64_t kFoldAlphaPairs = 1 << 22;
 
 static int finish_update_blocks(int64_t n) {
   const int64_t nb = (n + 8LL * kThreads - 1) / (8LL * kThreads);  // >= 8 elements per lane

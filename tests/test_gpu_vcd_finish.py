@@ -70,7 +70,7 @@ def _fused(L, c, prm, P, S, n_dir, n_fwd, alpha, alpha_sum, rows_buf):
 
 @pytest.mark.parametrize('P,S,n_sino,n_dir,n_fwd', [
     (300, 16, 50_000, 19, 96),        # alpha formed in every workgroup
-    (2048, 32, 400_000, 256, 5000),   # past the re-read limit: one alpha workgroup
+    (2048, 32, 4_000_000, 1024, 20000),  # past the re-read limit: one alpha workgroup
     (37, 5, 1_001, 1, 3),             # ragged: slices not a multiple of anything
 ])
 def test_finish_equals_separate_launches(P, S, n_sino, n_dir, n_fwd):

@@ -46,7 +46,8 @@ def test_size_queries_need_no_gpu():
                      delta_det_channel=1.0, delta_voxel=1.0, delta_voxel_row=1.0)
     assert lib.mbir_par_back_splits(g, 205012) == 1
     assert lib.mbir_par_back_splits(g, 1602) > 1
-    assert lib.mbir_par_fwd_partials_count(g, 0) == 720 * 2
+    # Both forward forms: one workgroup per (view, 128 channels, 64 rows).
+    assert lib.mbir_par_fwd_partials_count(g, 0) == 720 * 4
     assert lib.mbir_par_fwd_partials_count(g, 1) == 720 * 4
     assert lib.mbir_reduce_blocks(10 ** 6) >= 1
 
