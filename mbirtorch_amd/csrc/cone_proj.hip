@@ -112,7 +112,16 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
   const int lane = threadIdx.x & (MBIR_WAVE - 1), wave = threadIdx.x / MBIR_WAVE;
   const int slot = lane / RW, kl = lane % RW;
   const int S = g.num_slices, C = g.num_channels, R = g.num_det_rows;
-  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  // Split view ranges: XCD-aware order, one XCD runs consecutive pixel
+  // groups of one split (its sinogram lines stay in one L2).
+  int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  if (gridDim.y > 1) {
+    const int nx = gridDim.x, ny = gridDim.y;
+    const int l = xcd_remap(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
+    bx = l % nx;
+    by = (l / nx) % ny;
+    bz = l / (nx * ny);
+  }
   const int k = bz * RW + kl;
   const bool k_ok = k < S;
   const float kf = (float)k;

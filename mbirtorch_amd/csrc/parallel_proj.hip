@@ -109,7 +109,18 @@ __global__ __launch_bounds__(kBlock) void par_back_kernel(
   const int lane = threadIdx.x & (MBIR_WAVE - 1), wave = threadIdx.x / MBIR_WAVE;
   const int slot = lane / RW, rl = lane % RW;
   const int R = g.num_slices, C = g.num_channels;
-  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  // With the view range split (small pixel sets), XCD-aware order: one XCD
+  // runs consecutive pixel groups of one split, so the split's sinogram
+  // lines are fetched into one L2 rather than all eight (measured 1.4-1.7x
+  // at 1602-pixel subsets; a single split keeps the plain order).
+  int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  if (gridDim.y > 1) {
+    const int nx = gridDim.x, ny = gridDim.y;
+    const int l = xcd_remap(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
+    bx = l % nx;
+    by = (l / nx) % ny;
+    bz = l / (nx * ny);
+  }
   const int r = bz * RW + rl;
   const bool r_ok = r < R;
   const int p_base = bx * PIX_WG;
