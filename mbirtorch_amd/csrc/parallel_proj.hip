@@ -913,10 +913,12 @@ struct LaneCfg {
 };
 static inline bool fwd_vec(const mbir_par_geom& g) { return g.num_slices % 4 == 0; }
 // The ray form repeats a candidate's geometry on every lane of its unit, so
-// tall cylinders take eight rows per lane (fewer lanes per unit).  (Thinner
-// units -- two lanes of 32 rows, one channel -- issue fewer VALU
-// instructions but ~1.8x the cylinder loads, and measured 1.7x slower.)
+// tall cylinders take many rows per lane (fewer lanes per unit).  Measured at
+// 720 views x 64 x 512 on a 512x512x64 grid: 4 lanes x 16 rows x 2 channels
+// in batches of 3 candidates, 5.4 ms; 8 x 8 x 4 (batches of 4), 6.8 ms;
+// 2 x 32 x 2, 8.2 ms (register-bound).
 static inline LaneCfg ray_lanes(int R, bool vec) {
+  if (vec && R > 32 && R % 16 == 0) return {4, 16, 2};
   if (vec && R > 32 && R % 8 == 0) return {8, 8, 4};
   if (vec) {
     const int lpc = R <= 8 ? 2 : (R <= 16 ? 4 : (R <= 32 ? 8 : 16));
@@ -953,7 +955,7 @@ static long fwd_blocks(const mbir_par_geom& g, bool ray) {
 template <int LPC, int VR, int CPS, bool TWO, int MODE, bool WEIGHTED>
 static void launch_ray(const FwdArgs& a) {
   const mbir_par_geom& g = a.g;
-  constexpr int NB = 4;  // candidates per batch
+  constexpr int NB = VR >= 16 ? 3 : 4;  // candidates per batch (register budget)
   constexpr int CHW = (kBlock / LPC) * CPS;
   dim3 grid((g.num_channels + CHW - 1) / CHW, g.num_views,
             (g.num_slices + LPC * VR - 1) / (LPC * VR));
@@ -1013,6 +1015,7 @@ template <bool TWO>
 static int launch_fwd_lanes(const FwdArgs& a, bool vec) {
   if (a.runs) {
     const LaneCfg c = ray_lanes(a.g.num_slices, vec);
+    if (c.vr == 16) return launch_fwd_mode<4, 16, 2, TWO, true>(a);
     if (c.vr == 8) return launch_fwd_mode<8, 8, 4, TWO, true>(a);
     if (c.vr == 4) {
       switch (c.lpc) {
