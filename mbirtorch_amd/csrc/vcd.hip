@@ -301,7 +301,29 @@ struct FinishArgs {
   double* part;
 };
 
-template <typename TS>
+// Four consecutive sinogram entries (16-byte float / 8-byte bfloat16 words).
+__device__ __forceinline__ void ld4(const float* p, float (&v)[4]) {
+  const float4 t = *reinterpret_cast<const float4*>(p);
+  v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+}
+__device__ __forceinline__ void ld4(const bf16_bits* p, float (&v)[4]) {
+  const uint2 t = *reinterpret_cast<const uint2*>(p);
+  v[0] = __uint_as_float(t.x << 16);
+  v[1] = __uint_as_float(t.x & 0xffff0000u);
+  v[2] = __uint_as_float(t.y << 16);
+  v[3] = __uint_as_float(t.y & 0xffff0000u);
+}
+__device__ __forceinline__ void st4(float* p, const float (&v)[4]) {
+  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void st4(bf16_bits* p, const float (&v)[4]) {
+  uint2 t;
+  t.x = (uint32_t)f32_to_bf16(v[0]) | ((uint32_t)f32_to_bf16(v[1]) << 16);
+  t.y = (uint32_t)f32_to_bf16(v[2]) | ((uint32_t)f32_to_bf16(v[3]) << 16);
+  *reinterpret_cast<uint2*>(p) = t;
+}
+
+template <typename TS, bool VEC>
 __global__ __launch_bounds__(kThreads) void vcd_finish_kernel(FinishArgs a) {
   float al;
   if (a.dir_parts) {
@@ -315,9 +337,21 @@ __global__ __launch_bounds__(kThreads) void vcd_finish_kernel(FinishArgs a) {
   }
   if ((int)blockIdx.x < a.nb_update) {
     TS* e = static_cast<TS*>(a.e);
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < a.n_sino;
-         i += (int64_t)a.nb_update * kThreads)
-      st_s(e + i, ld_s(e + i) - al * a.adelta[i]);
+    const int64_t stride = (int64_t)a.nb_update * kThreads;
+    if (VEC) {  // n_sino % 4 == 0, aligned rows
+      for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < a.n_sino / 4;
+           i += stride) {
+        float ev[4], dv[4];
+        ld4(e + 4 * i, ev);
+        ld4(a.adelta + 4 * i, dv);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ev[q] = ev[q] - al * dv[q];
+        st4(e + 4 * i, ev);
+      }
+    } else {
+      for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < a.n_sino; i += stride)
+        st_s(e + i, ld_s(e + i) - al * a.adelta[i]);
+    }
     return;  // the whole workgroup leaves: no barrier is skipped
   }
   vcd_apply_block<true>(a.x, a.pix, a.P, a.S, a.delta, al, a.ppb, blockIdx.x - a.nb_update,
@@ -512,9 +546,10 @@ int64_t mbir_vcd_apply_scratch(int32_t P, int32_t S) {
 // forming alpha in every workgroup (64 MiB of re-read partials, from L2).
 constexpr int64_t kFoldAlphaPairs = 1 << 22;
 
-static int finish_update_blocks(int64_t n) {
-  const int64_t nb = (n + 8LL * kThreads - 1) / (8LL * kThreads);  // >= 8 elements per lane
-  return (int)(nb > 512 ? 512 : nb);
+// Update workgroups: >= 16 entries per lane, at most `cap` workgroups.
+static int finish_update_blocks(int64_t n, int cap) {
+  const int64_t nb = (n + 16LL * kThreads - 1) / (16LL * kThreads);
+  return (int)(nb > cap ? cap : nb);
 }
 
 int32_t mbir_vcd_block_stats_rows(int32_t P) { return P > 0 ? apply_blocks(P) : 0; }
@@ -534,7 +569,7 @@ static int vcd_finish_any(void* e, const float* adelta, int64_t n_sino, float* x
   a.e = e;
   a.adelta = adelta;
   a.n_sino = n_sino;
-  a.nb_update = finish_update_blocks(n_sino);
+  a.nb_update = finish_update_blocks(n_sino, 1024);
   a.x = x;
   a.pix = pix;
   a.P = P;
@@ -555,16 +590,25 @@ static int vcd_finish_any(void* e, const float* adelta, int64_t n_sino, float* x
   const int64_t grid = (int64_t)a.nb_update + nb_a;
   if (dir_parts && (int64_t)(n_dir + n_fwd) * grid > kFoldAlphaPairs) {
     // Every workgroup would re-read too many partials: one workgroup forms
-    // alpha first (the same arithmetic), the finishing launch reads it.
+    // alpha first (the same arithmetic), the finishing launch reads it with
+    // more update workgroups.
     vcd_alpha_kernel<<<1, kThreads, 0, s>>>(dir_parts, n_dir, fwd_parts, n_fwd, a.fm,
                                             a.max_alpha, alpha, alpha_sum);
     MBIR_CHECK_LAUNCH();
     a.dir_parts = nullptr;
+    a.nb_update = finish_update_blocks(n_sino, 8192);
   }
-  if (bf16)
-    vcd_finish_kernel<bf16_bits><<<a.nb_update + nb_a, kThreads, 0, s>>>(a);
-  else
-    vcd_finish_kernel<float><<<a.nb_update + nb_a, kThreads, 0, s>>>(a);
+  const size_t wd = bf16 ? 8 : 16;  // bytes of four entries
+  const bool vec = n_sino % 4 == 0 && reinterpret_cast<uintptr_t>(e) % wd == 0 &&
+                   reinterpret_cast<uintptr_t>(adelta) % 16 == 0;
+  const int nb = a.nb_update + nb_a;
+  if (bf16) {
+    if (vec) vcd_finish_kernel<bf16_bits, true><<<nb, kThreads, 0, s>>>(a);
+    else vcd_finish_kernel<bf16_bits, false><<<nb, kThreads, 0, s>>>(a);
+  } else {
+    if (vec) vcd_finish_kernel<float, true><<<nb, kThreads, 0, s>>>(a);
+    else vcd_finish_kernel<float, false><<<nb, kThreads, 0, s>>>(a);
+  }
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }

@@ -71,7 +71,7 @@ def _fused(L, c, prm, P, S, n_dir, n_fwd, alpha, alpha_sum, rows_buf):
 @pytest.mark.parametrize('P,S,n_sino,n_dir,n_fwd', [
     (300, 16, 50_000, 19, 96),        # alpha formed in every workgroup
     (2048, 32, 4_000_000, 1024, 20000),  # past the re-read limit: one alpha workgroup
-    (37, 5, 1_001, 1, 3),             # ragged: slices not a multiple of anything
+    (37, 5, 1_001, 1, 3),             # ragged: entry-wise update path
 ])
 def test_finish_equals_separate_launches(P, S, n_sino, n_dir, n_fwd):
     L = _lib()
@@ -109,14 +109,15 @@ def test_finish_equals_separate_launches(P, S, n_sino, n_dir, n_fwd):
     assert torch.count_nonzero(rows_buf).item() == 0  # the reduction clears the rows
 
 
-def test_finish_bf16_and_read_alpha():
+@pytest.mark.parametrize('n', [30_001, 30_000])  # scalar and four-entry update paths
+def test_finish_bf16_and_read_alpha(n):
     """The bf16 error update rounds like mbir_vcd_error_update_bf16, and
     dir_parts = NULL with n_sino = 0 is a plain recon update with alpha read
     from the device."""
     L = _lib()
     lib, p, st = L.lib(), L.ptr, L.stream_ptr()
     from mbirtorch_amd.tomography_model import _to_bf16
-    P, S, n = 500, 8, 30_001
+    P, S = 500, 8
     prm = _params(L, S)
     c = _case(np.random.default_rng(7), P, S, n, 4, 40)
     alpha = torch.full((1,), 0.37, dtype=torch.float32, device='cuda')
