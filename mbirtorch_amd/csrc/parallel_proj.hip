@@ -543,6 +543,72 @@ __device__ __forceinline__ void ray_taps(const int (&ci)[NB], const int (&cj)[NB
       for (int q = 0; q < VR; ++q) acc[k][q] = fmaf(a[u][k], x[u][q], acc[k][q]);
 }
 
+// Four-lane units: each lane evaluates one candidate of a batch of four
+// (n_p and its CPS weights, the list position, live or not), and the unit's
+// lanes exchange them with quad-permute DPP moves, so the geometry is
+// computed once per candidate instead of once per lane.  The loads of the
+// batch go out in PH phases of 4/PH candidates (register budget).
+template <int U>
+__device__ __forceinline__ int quad_bcast(int x) {
+  return __builtin_amdgcn_mov_dpp(x, U | (U << 2) | (U << 4) | (U << 6), 0xF, 0xF, false);
+}
+template <int U>
+__device__ __forceinline__ float quad_bcast(float x) {
+  return __int_as_float(quad_bcast<U>(__float_as_int(x)));
+}
+
+template <int U, int CPS>
+__device__ __forceinline__ void quad_take(const float (&a)[CPS], int pos, int use,
+                                          float (&A)[4][CPS], int (&POS)[4], int (&USE)[4]) {
+#pragma unroll
+  for (int k = 0; k < CPS; ++k) A[U][k] = quad_bcast<U>(a[k]);
+  POS[U] = quad_bcast<U>(pos);
+  USE[U] = quad_bcast<U>(use);
+}
+
+template <int VR, int CPS, int PH>
+__device__ __forceinline__ void quad_ray_taps(int ci, int cj, int pos, bool ok, int n0,
+                                              bool row_ok, int rr, const ParView& vw,
+                                              const mbir_par_geom& g,
+                                              const float* __restrict__ values,
+                                              float (&acc)[CPS][VR]) {
+  const float n_p = par_np(ci, cj, vw, g);
+  float a[CPS];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < CPS; ++k) {
+    a[k] = par_weight(n_p, n0 + k, vw);
+    any = any || a[k] != 0.0f;
+  }
+  float A[4][CPS];
+  int POS[4], USE[4];
+  const int use = (ok && any) ? 1 : 0;
+  quad_take<0, CPS>(a, pos, use, A, POS, USE);
+  quad_take<1, CPS>(a, pos, use, A, POS, USE);
+  quad_take<2, CPS>(a, pos, use, A, POS, USE);
+  quad_take<3, CPS>(a, pos, use, A, POS, USE);
+  constexpr int PER = 4 / PH;
+#pragma unroll
+  for (int h = 0; h < PH; ++h) {
+    float x[PER][VR];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      if (USE[h * PER + u] && row_ok) {
+        load_vr<VR>(values + (size_t)POS[h * PER + u] * g.num_slices + rr, x[u]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < VR; ++q) x[u][q] = 0.0f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+#pragma unroll
+      for (int k = 0; k < CPS; ++k)
+#pragma unroll
+        for (int q = 0; q < VR; ++q) acc[k][q] = fmaf(A[h * PER + u][k], x[u][q], acc[k][q]);
+  }
+}
+
 template <int LPC, int VR, int CPS, int NB, bool TWO, int MODE, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
     const float* __restrict__ values, const int* __restrict__ runs,
@@ -552,6 +618,10 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
   using L = FwdLanes<LPC, VR>;
   constexpr int CHW = L::UNITS * CPS;
   constexpr int TS = CHW + 1;
+  // Four-lane units with NB a multiple of 4 share candidate geometry across
+  // the unit (NB / 4 load phases per batch of four).
+  constexpr bool QUAD = LPC == 4 && NB % 4 == 0;
+  constexpr int QPH = QUAD ? NB / 4 : 1;
   __shared__ float s_tile[MODE == FWD_WRITE_VRC ? L::ROWS * TS : 1];
 
   const int lane = threadIdx.x & (MBIR_WAVE - 1), wave = threadIdx.x / MBIR_WAVE;
@@ -584,49 +654,71 @@ __global__ __launch_bounds__(kBlock) void par_fwd_ray_kernel(
   const float xa = ((float)n0 - g.det_center) * ddc - g.det_channel_offset;
   const float xb = ((float)(n0 + CPS - 1) - g.det_center) * ddc - g.det_channel_offset;
   const float cd = vw.cosv * g.delta_voxel, sd = vw.sinv * g.delta_voxel_row;
+  // The candidate ranges use reciprocals: their rounding (~1e-7 relative) is
+  // far inside the 0.01-pixel margin; each candidate's n_p is exact.
   if (fabsf(cd) >= fabsf(sd)) {
     const float hj = vw.halfw * ddc / fabsf(cd) + kRayMargin;
+    const float icd = 1.0f / cd;
     for (int i = 0; i < Nr; ++i) {
       const int start = runs[3 * i], jmin = runs[3 * i + 1], cnt = runs[3 * i + 2];
       if (cnt <= 0) continue;
       const float sy = vw.sinv * (g.delta_voxel_row * ((float)i - g.row_center));
-      const float ja = (xa + sy) / cd + g.col_center, jb = (xb + sy) / cd + g.col_center;
+      const float ja = (xa + sy) * icd + g.col_center, jb = (xb + sy) * icd + g.col_center;
       const int jlo = max((int)floorf(fminf(ja, jb) - hj), jmin);
       const int jhi = min((int)ceilf(fmaxf(ja, jb) + hj), jmin + cnt - 1);
-      for (int j = jlo; j <= jhi; j += NB) {
-        int ci[NB], cj[NB], pos[NB];
-        bool ok[NB];
-#pragma unroll
-        for (int u = 0; u < NB; ++u) {
-          const int jj = min(j + u, jhi);
-          ci[u] = i;
-          cj[u] = jj;
-          pos[u] = start + jj - jmin;
-          ok[u] = j + u <= jhi;
+      if constexpr (QUAD) {
+        for (int j = jlo; j <= jhi; j += 4) {
+          const int jj = min(j + rl, jhi);
+          quad_ray_taps<VR, CPS, QPH>(i, jj, start + jj - jmin, j + rl <= jhi, n0, row_ok, rr,
+                                      vw, g, values, acc);
         }
-        ray_taps<NB, VR, CPS, TWO>(ci, cj, pos, ok, n0, ntaps, row_ok, rr, vw, g, values, acc);
+      } else {
+        for (int j = jlo; j <= jhi; j += NB) {
+          int ci[NB], cj[NB], pos[NB];
+          bool ok[NB];
+#pragma unroll
+          for (int u = 0; u < NB; ++u) {
+            const int jj = min(j + u, jhi);
+            ci[u] = i;
+            cj[u] = jj;
+            pos[u] = start + jj - jmin;
+            ok[u] = j + u <= jhi;
+          }
+          ray_taps<NB, VR, CPS, TWO>(ci, cj, pos, ok, n0, ntaps, row_ok, rr, vw, g, values, acc);
+        }
       }
     }
   } else {
     const float hi = vw.halfw * ddc / fabsf(sd) + kRayMargin;
+    const float isd = 1.0f / sd;
     for (int j = 0; j < Nc; ++j) {
       const float cx = vw.cosv * (g.delta_voxel * ((float)j - g.col_center));
-      const float ia = (cx - xa) / sd + g.row_center, ib = (cx - xb) / sd + g.row_center;
+      const float ia = (cx - xa) * isd + g.row_center, ib = (cx - xb) * isd + g.row_center;
       const int ilo = max((int)floorf(fminf(ia, ib) - hi), 0);
       const int ihi = min((int)ceilf(fmaxf(ia, ib) + hi), Nr - 1);
-      for (int i = ilo; i <= ihi; i += NB) {
-        int ci[NB], cj[NB], pos[NB];
-        bool ok[NB];
-#pragma unroll
-        for (int u = 0; u < NB; ++u) {
-          const int ii = min(i + u, ihi);
+      if constexpr (QUAD) {
+        for (int i = ilo; i <= ihi; i += 4) {
+          const int ii = min(i + rl, ihi);
           const int start = runs[3 * ii], jmin = runs[3 * ii + 1], cnt = runs[3 * ii + 2];
-          ci[u] = ii;
-          cj[u] = j;
-          pos[u] = start + j - jmin;
-          ok[u] = i + u <= ihi && j >= jmin && j < jmin + cnt;
+          quad_ray_taps<VR, CPS, QPH>(ii, j, start + j - jmin,
+                                      i + rl <= ihi && j >= jmin && j < jmin + cnt, n0, row_ok,
+                                      rr, vw, g, values, acc);
         }
-        ray_taps<NB, VR, CPS, TWO>(ci, cj, pos, ok, n0, ntaps, row_ok, rr, vw, g, values, acc);
+      } else {
+        for (int i = ilo; i <= ihi; i += NB) {
+          int ci[NB], cj[NB], pos[NB];
+          bool ok[NB];
+#pragma unroll
+          for (int u = 0; u < NB; ++u) {
+            const int ii = min(i + u, ihi);
+            const int start = runs[3 * ii], jmin = runs[3 * ii + 1], cnt = runs[3 * ii + 2];
+            ci[u] = ii;
+            cj[u] = j;
+            pos[u] = start + j - jmin;
+            ok[u] = i + u <= ihi && j >= jmin && j < jmin + cnt;
+          }
+          ray_taps<NB, VR, CPS, TWO>(ci, cj, pos, ok, n0, ntaps, row_ok, rr, vw, g, values, acc);
+        }
       }
     }
   }
@@ -1083,14 +1175,14 @@ struct LaneCfg {
   int lpc, vr, cps;
 };
 static inline bool fwd_vec(const mbir_par_geom& g) { return g.num_slices % 4 == 0; }
-// The ray form repeats a candidate's geometry on every lane of its unit, so
-// tall cylinders take many rows per lane (fewer lanes per unit).  Measured at
-// 720 views x 64 x 512 on a 512x512x64 grid: 4 lanes x 16 rows x 2 channels
-// in batches of 3 candidates, 5.4 ms; 8 x 8 x 4 (batches of 4), 6.8 ms;
-// 2 x 32 x 2, 8.2 ms (register-bound).
+// The ray form's four-lane units evaluate each candidate's geometry once and
+// share it over quad DPP moves; the rest of a candidate's cost is its VR x CPS
+// FMAs per lane.  Measured at 720 views x 64 x 512 on a 512x512x64 grid:
+// 4 lanes x 8 rows x 4 channels 4.2 ms; x 16 rows x 4 channels 4.4 ms; x 16
+// rows x 2 channels 4.8 ms (5.4 ms before the geometry sharing); 8 x 8 x 4
+// without sharing 6.8 ms.
 static inline LaneCfg ray_lanes(int R, bool vec) {
-  if (vec && R > 32 && R % 16 == 0) return {4, 16, 2};
-  if (vec && R > 32 && R % 8 == 0) return {8, 8, 4};
+  if (vec && R >= 24 && R % 8 == 0) return {4, 8, 4};
   if (vec) {
     const int lpc = R <= 8 ? 2 : (R <= 16 ? 4 : (R <= 32 ? 8 : 16));
     return {lpc, 4, lpc >= 8 ? 4 : (lpc == 4 ? 2 : 1)};
@@ -1126,7 +1218,7 @@ static long fwd_blocks(const mbir_par_geom& g, bool ray) {
 template <int LPC, int VR, int CPS, bool TWO, int MODE, bool WEIGHTED>
 static void launch_ray(const FwdArgs& a) {
   const mbir_par_geom& g = a.g;
-  constexpr int NB = VR >= 16 ? 3 : 4;  // candidates per batch (register budget)
+  constexpr int NB = 4;  // candidates per batch (one per lane of a four-lane unit)
   constexpr int CHW = (kBlock / LPC) * CPS;
   dim3 grid((g.num_channels + CHW - 1) / CHW, g.num_views,
             (g.num_slices + LPC * VR - 1) / (LPC * VR));
@@ -1186,8 +1278,7 @@ template <bool TWO>
 static int launch_fwd_lanes(const FwdArgs& a, bool vec) {
   if (a.runs) {
     const LaneCfg c = ray_lanes(a.g.num_slices, vec);
-    if (c.vr == 16) return launch_fwd_mode<4, 16, 2, TWO, true>(a);
-    if (c.vr == 8) return launch_fwd_mode<8, 8, 4, TWO, true>(a);
+    if (c.vr == 8) return launch_fwd_mode<4, 8, 4, TWO, true>(a);
     if (c.vr == 4) {
       switch (c.lpc) {
         case 2: return launch_fwd_mode<2, 4, 1, TWO, true>(a);
