@@ -167,9 +167,13 @@ int32_t mbir_par_fwd_partials_count(const mbir_par_geom* geom, int32_t ray);
  * search reductions lin = sum(w * e * A delta), quad = sum(w * (A delta)^2)
  * (w == NULL means unit weights), FP64, one (lin, quad) pair per workgroup
  * into partials[2 * mbir_par_fwd_partials_count(geom, row_runs != NULL)].
- * `row_runs` as for mbir_par_forward.  With adelta_vcr non-NULL, A delta is
- * also stored there (channel-major, float32) for mbir_vcd_error_update.  Replaces sparse_forward_project(delta) +
- * _forward_lin_quad_const/_weighted (tomography_model.py:2592-2610, 65-74). */
+ * error_vcr == NULL leaves lin at 0 and reads no error sinogram: the caller
+ * takes <W e, A delta> = <A^T W e, delta> from the back projection (the
+ * third value of each mbir_vcd_direction partial).  `row_runs` as for
+ * mbir_par_forward.  With adelta_vcr non-NULL, A delta is also stored there
+ * (channel-major, float32) for mbir_vcd_error_update.  Replaces
+ * sparse_forward_project(delta) + _forward_lin_quad_const/_weighted
+ * (tomography_model.py:2592-2610, 65-74). */
 int mbir_par_fwd_linesearch(const float* delta, const int64_t* pixel_indices,
                             int32_t num_pixels, const int32_t* row_runs,
                             const float* view_table, const mbir_par_geom* geom,
@@ -255,8 +259,10 @@ int32_t mbir_vcd_direction_partials_count(int32_t num_pixels,
 /* Prior gradient/Hessian at the subset (qGGMRF surrogate, or the proximal
  * prior when prm->use_prox), forward gradient -fm*sum(back partials) and
  * Hessian fm*fm_hessian[pix], the direction delta = -(g_f+g_p)/(H_f+H_p),
- * and the prior line terms (sum g_p*delta, sum H_p*delta^2) as FP64 partial
- * pairs.  Replaces qggmrf_gradient_and_hessian_at_indices
+ * and FP64 partial triples per workgroup (sum g_p*delta, sum H_p*delta^2,
+ * sum b*delta with b = sum of the back partials = A^T W e at the subset --
+ * the line search's linear term, so the forward line search can skip the
+ * error sinogram).  Replaces qggmrf_gradient_and_hessian_at_indices
  * (qggmrf.py:77-183), prox_gradient_at_indices (qggmrf.py:186-206),
  * _diagonal_update_direction and _prior_line_terms
  * (tomography_model.py:54-62, 2520-2576). */
@@ -266,7 +272,8 @@ int mbir_vcd_direction(const float* flat_recon, const int64_t* pixel_indices,
                        const float* prox_input, const mbir_vcd_params* prm,
                        float* delta, double* partials, mbir_stream_t stream);
 
-/* Scratch floats mbir_vcd_direction_dc needs: 4 * num_pixels * num_slices. */
+/* Scratch floats mbir_vcd_direction_dc needs: 5 * num_pixels * num_slices
+ * plus the per-slice means. */
 int64_t mbir_vcd_direction_dc_scratch(int32_t num_pixels, int32_t num_slices);
 
 /* Cone DC-damped direction (cone_beam.py:48-55, 476-496):
@@ -282,8 +289,12 @@ int mbir_vcd_direction_dc(const float* flat_recon, const int64_t* pixel_indices,
                           double* partials, mbir_stream_t stream);
 
 /* alpha = clamp((fm*lin - prior_lin) / (fm*quad + prior_quad + eps32),
- * eps32, max_alpha), reduced in a fixed order; writes alpha[0] (float32)
- * and adds it to alpha_sum[0] (FP64).  Replaces tomography_model.py:2612-2617. */
+ * eps32, max_alpha), reduced in a fixed order from the direction triples
+ * (prior_lin, prior_quad, back_lin) and the line-search pairs (fwd_lin,
+ * quad), with lin = fwd_lin + back_lin (exactly one of the two is nonzero: a
+ * line search run with the error sinogram must be paired with direction
+ * triples whose third values are zero); writes alpha[0] (float32) and adds
+ * it to alpha_sum[0] (FP64).  Replaces tomography_model.py:2612-2617. */
 int mbir_vcd_alpha(const double* prior_parts, int32_t num_prior,
                    const double* fwd_parts, int32_t num_fwd,
                    const mbir_vcd_params* prm, float* alpha,
@@ -375,7 +386,7 @@ int mbir_cone_fwd_update_bf16(const float* delta, const int64_t* pixel_indices,
  * its partition subsets (subset id -> device index array, size, optional
  * image-row run table).  Scratch sizes: back_parts >= back_parts_capacity
  * floats (max over subsets of splits * size * slices), delta >= max size *
- * slices floats, dir_parts >= 2 * mbir_vcd_direction_partials_count,
+ * slices floats, dir_parts >= 3 * mbir_vcd_direction_partials_count,
  * fwd_parts >= 2 * max forward partials count, apply_scratch >=
  * mbir_vcd_apply_scratch(max size, slices) doubles and zero before the first
  * run (it carries per-workgroup statistics to the end of each iteration,

@@ -430,13 +430,14 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_bin_kernel(
         if constexpr (sizeof(TS) == sizeof(float)) sino[o] = t;
       } else if (MODE == CF_LINESEARCH) {
         if (adelta) adelta[o] = t;  // A(delta) kept for the error update
-        const float e = ld_s(sino + o);
+        // No error sinogram: the linear term comes from the back projection.
+        const float e = sino ? ld_s(sino + o) : 0.0f;
         if (WEIGHTED) {
           const float w = ld_s(wts + o);
-          lin += (double)((w * e) * t);
+          if (sino) lin += (double)((w * e) * t);
           quad += (double)((t * t) * w);
         } else {
-          lin += (double)(e * t);
+          if (sino) lin += (double)(e * t);
           quad += (double)(t * t);
         }
       } else {
@@ -668,7 +669,7 @@ static int cone_ls_any(const float* delta, const int64_t* pix, int32_t P, const 
                        float* adelta, double* partials, mbir_stream_t stream, bool bf16) {
   int st = check_cone(g);
   if (st) return st;
-  if (!delta || !pix || !view_table || !error_vcr || !partials || P < 0) return MBIR_ERR_ARG;
+  if (!delta || !pix || !view_table || !partials || P < 0) return MBIR_ERR_ARG;
   ConeFwdArgs a{CF_LINESEARCH, delta, pix, P, reinterpret_cast<const ConeView*>(view_table), *g,
                 const_cast<void*>(error_vcr), w, nullptr, partials, (hipStream_t)stream, bf16,
                 adelta};

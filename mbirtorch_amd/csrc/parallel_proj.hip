@@ -429,20 +429,27 @@ __device__ __forceinline__ void fwd_epilogue_vcr(const float (&t)[VR], size_t o,
     if constexpr (sizeof(TS) == sizeof(float)) store_vr<VR>(sino + o, t);
   } else if (MODE == FWD_LINESEARCH) {
     if (adelta) store_vr<VR>(adelta + o, t);  // A(delta) kept for the error update
+    // No error sinogram: the linear term <W e, A delta> comes from the back
+    // projection instead, and only the quadratic term is formed here.
     float e[VR];
-    load_vr<VR>(sino + o, e);
+    if (sino) {
+      load_vr<VR>(sino + o, e);
+    } else {
+#pragma unroll
+      for (int q = 0; q < VR; ++q) e[q] = 0.0f;
+    }
     if (WEIGHTED) {
       float w[VR];
       load_vr<VR>(wts + o, w);
 #pragma unroll
       for (int q = 0; q < VR; ++q) {
-        lin += (double)((w[q] * e[q]) * t[q]);
+        if (sino) lin += (double)((w[q] * e[q]) * t[q]);
         quad += (double)((t[q] * t[q]) * w[q]);
       }
     } else {
 #pragma unroll
       for (int q = 0; q < VR; ++q) {
-        lin += (double)(e[q] * t[q]);
+        if (sino) lin += (double)(e[q] * t[q]);
         quad += (double)(t[q] * t[q]);
       }
     }
@@ -1410,8 +1417,7 @@ int mbir_par_fwd_linesearch(const float* delta, const int64_t* pix, int32_t P,
                             float* adelta_vcr, double* partials, mbir_stream_t stream) {
   int st = check_geom(g);
   if (st) return st;
-  if (!delta || !view_table || !error_vcr || !partials || P < 0 || (!pix && !runs))
-    return MBIR_ERR_ARG;
+  if (!delta || !view_table || !partials || P < 0 || (!pix && !runs)) return MBIR_ERR_ARG;
   FwdArgs a{FWD_LINESEARCH, delta, pix, P, runs,
             reinterpret_cast<const ParView*>(view_table), *g, const_cast<float*>(error_vcr), w,
             nullptr, partials, (hipStream_t)stream, false, adelta_vcr};
@@ -1436,7 +1442,7 @@ int mbir_par_fwd_linesearch_bf16(const float* delta, const int64_t* pix, int32_t
                                  double* partials, mbir_stream_t stream) {
   int st = check_geom(g);
   if (st) return st;
-  if (!delta || !pix || !view_table || !error_vcr || !partials || P < 0) return MBIR_ERR_ARG;
+  if (!delta || !pix || !view_table || !partials || P < 0) return MBIR_ERR_ARG;
   FwdArgs a{FWD_LINESEARCH, delta, pix, P, nullptr,
             reinterpret_cast<const ParView*>(view_table), *g, const_cast<uint16_t*>(error_vcr),
             w, nullptr, partials, (hipStream_t)stream, true, adelta_vcr};

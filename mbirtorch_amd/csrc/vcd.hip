@@ -34,7 +34,7 @@ __device__ __forceinline__ void vcd_terms(const float* __restrict__ x,
                                           const float* __restrict__ hess,
                                           const float* __restrict__ prox,
                                           const mbir_vcd_params& prm, float& fg, float& gp,
-                                          float& fh, float& hp) {
+                                          float& fh, float& hp, float& bsum_out) {
   const int S = prm.num_slices, Nc = prm.num_cols, Nr = prm.num_rows;
   const int p = (int)(i / S), s = (int)(i - (int64_t)p * S);
   const int64_t pi = pix[p];
@@ -77,6 +77,7 @@ __device__ __forceinline__ void vcd_terms(const float* __restrict__ x,
   for (; k < nsplit; ++k) bsum += back_parts[(size_t)k * total + i];
   fg = prm.neg_fm_constant * bsum;
   fh = prm.fm_constant * hess[pi * S + s];
+  bsum_out = bsum;
 }
 
 __global__ __launch_bounds__(kThreads) void vcd_direction_kernel(
@@ -85,20 +86,23 @@ __global__ __launch_bounds__(kThreads) void vcd_direction_kernel(
     const float* __restrict__ prox, mbir_vcd_params prm, float* __restrict__ delta,
     double* __restrict__ partials) {
   const int64_t total = (int64_t)P * prm.num_slices;
-  double pl = 0.0, pq = 0.0;
+  double pl = 0.0, pq = 0.0, bl = 0.0, unused = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * kThreads) {
-    float fg, gp, fh, hp;
-    vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp);
+    float fg, gp, fh, hp, bs;
+    vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp, bs);
     const float d = -((fg + gp) / (fh + hp));
     delta[i] = d;
     pl += (double)(gp * d);
     pq += (double)((hp * d) * d);
+    bl += (double)(bs * d);
   }
   block_sum2_f64(pl, pq);
+  block_sum2_f64(bl, unused);
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = pl;
-    partials[2 * blockIdx.x + 1] = pq;
+    partials[3 * blockIdx.x] = pl;
+    partials[3 * blockIdx.x + 1] = pq;
+    partials[3 * blockIdx.x + 2] = bl;
   }
 }
 
@@ -110,12 +114,13 @@ __global__ __launch_bounds__(kThreads) void vcd_dc_terms_kernel(
   const int64_t total = (int64_t)P * prm.num_slices;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * kThreads) {
-    float fg, gp, fh, hp;
-    vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp);
+    float fg, gp, fh, hp, bs;
+    vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp, bs);
     scratch[i] = fg + gp;                   // g
     scratch[total + i] = 1.0f / (fh + hp);  // h_inv
     scratch[2 * total + i] = gp;
     scratch[3 * total + i] = hp;
+    scratch[4 * total + i] = bs;            // A^T W e
   }
 }
 
@@ -142,7 +147,7 @@ __global__ __launch_bounds__(kThreads) void vcd_dc_delta_kernel(
     const float* __restrict__ prof, int P, int S, float* __restrict__ delta,
     double* __restrict__ partials) {
   const int64_t total = (int64_t)P * S;
-  double pl = 0.0, pq = 0.0;
+  double pl = 0.0, pq = 0.0, bl = 0.0, unused = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * kThreads) {
     const int s = (int)(i % S);
@@ -153,11 +158,14 @@ __global__ __launch_bounds__(kThreads) void vcd_dc_delta_kernel(
     delta[i] = d;
     pl += (double)(gp * d);
     pq += (double)((hp * d) * d);
+    bl += (double)(scratch[4 * total + i] * d);
   }
   block_sum2_f64(pl, pq);
+  block_sum2_f64(bl, unused);
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = pl;
-    partials[2 * blockIdx.x + 1] = pq;
+    partials[3 * blockIdx.x] = pl;
+    partials[3 * blockIdx.x + 1] = pq;
+    partials[3 * blockIdx.x + 2] = bl;
   }
 }
 
@@ -168,12 +176,20 @@ __device__ float vcd_alpha_value(const double* __restrict__ pp, int npp,
                                  const double* __restrict__ fp, int nfp, double fm,
                                  double max_alpha) {
   __shared__ float s_alpha;
-  double pl = 0.0, pq = 0.0, lin = 0.0, quad = 0.0;
-  for (int i = threadIdx.x; i < npp; i += kThreads) { pl += pp[2 * i]; pq += pp[2 * i + 1]; }
+  double pl = 0.0, pq = 0.0, bl = 0.0, lin = 0.0, quad = 0.0, unused = 0.0;
+  for (int i = threadIdx.x; i < npp; i += kThreads) {
+    pl += pp[3 * i];
+    pq += pp[3 * i + 1];
+    bl += pp[3 * i + 2];
+  }
   for (int i = threadIdx.x; i < nfp; i += kThreads) { lin += fp[2 * i]; quad += fp[2 * i + 1]; }
   block_sum2_f64(pl, pq);
   block_sum2_f64(lin, quad);
+  block_sum2_f64(bl, unused);
   if (threadIdx.x == 0) {
+    // <W e, A delta>: from the line search, or (when it ran without the
+    // error sinogram) as <A^T W e, delta> from the direction partials.
+    lin = lin + bl;
     const double eps = (double)kEps32;
     double a = (fm * lin - pl) / (fm * quad + pq + eps);
     if (!(a >= eps)) a = eps;  // also maps NaN to eps, like a clamp of a NaN-free value
@@ -486,8 +502,8 @@ int mbir_vcd_direction(const float* x, const int64_t* pix, int32_t P, const floa
 }
 
 int64_t mbir_vcd_direction_dc_scratch(int32_t P, int32_t S) {
-  // 4 float arrays of P*S plus S doubles (as 2*S floats), rounded up.
-  return 4 * (int64_t)P * S + 2 * (int64_t)S + 4;
+  // 5 float arrays of P*S plus S doubles (as 2*S floats), rounded up.
+  return 5 * (int64_t)P * S + 2 * (int64_t)S + 4;
 }
 
 int mbir_vcd_direction_dc(const float* x, const int64_t* pix, int32_t P, const float* back_parts,
@@ -502,8 +518,8 @@ int mbir_vcd_direction_dc(const float* x, const int64_t* pix, int32_t P, const f
   const int S = prm->num_slices;
   const int nb = mbir_vcd_direction_partials_count(P, S);
   const int64_t total = (int64_t)P * S;
-  // gbar lives after the four float arrays, 8-byte aligned.
-  uintptr_t gaddr = reinterpret_cast<uintptr_t>(scratch + 4 * total);
+  // gbar lives after the five float arrays, 8-byte aligned.
+  uintptr_t gaddr = reinterpret_cast<uintptr_t>(scratch + 5 * total);
   gaddr = (gaddr + 7) & ~(uintptr_t)7;
   double* gbar = reinterpret_cast<double*>(gaddr);
   vcd_dc_terms_kernel<<<nb, kThreads, 0, s>>>(x, pix, P, back_parts, nsplit, hess, prox, *prm,

@@ -74,23 +74,26 @@ int subset_step(const mbir_vcd_job* j, int k, double* stats, hipStream_t s) {
   else
     MBIR_TRY(mbir_vcd_direction(j->flat_recon, pix, P, j->back_parts, ns, j->hessian, j->prox,
                                 &j->prm, j->delta, j->dir_parts, s));
+  // The line search forms only sum w (A delta)^2: the linear term
+  // <W e, A delta> = <A^T W e, delta> came with the direction partials from
+  // the back projection, so the error sinogram is not read here.
   int n_fwd;
   float* adelta = j->positivity ? nullptr : j->adelta;
   if (par) {
     if (h)
-      MBIR_TRY(mbir_par_fwd_linesearch_bf16(j->delta, pix, P, j->view_table, j->par_geom, e16,
-                                            w16, adelta, j->fwd_parts, s));
+      MBIR_TRY(mbir_par_fwd_linesearch_bf16(j->delta, pix, P, j->view_table, j->par_geom,
+                                            nullptr, w16, adelta, j->fwd_parts, s));
     else
       MBIR_TRY(mbir_par_fwd_linesearch(j->delta, pix, P, runs, j->view_table, j->par_geom,
-                                       j->error_vcr, j->weights_vcr, adelta, j->fwd_parts, s));
+                                       nullptr, j->weights_vcr, adelta, j->fwd_parts, s));
     n_fwd = mbir_par_fwd_partials_count(j->par_geom, runs != nullptr);
   } else {
     if (h)
-      MBIR_TRY(mbir_cone_fwd_linesearch_bf16(j->delta, pix, P, j->view_table, j->cone_geom, e16,
-                                             w16, adelta, j->fwd_parts, s));
+      MBIR_TRY(mbir_cone_fwd_linesearch_bf16(j->delta, pix, P, j->view_table, j->cone_geom,
+                                             nullptr, w16, adelta, j->fwd_parts, s));
     else
-      MBIR_TRY(mbir_cone_fwd_linesearch(j->delta, pix, P, j->view_table, j->cone_geom,
-                                        j->error_vcr, j->weights_vcr, adelta, j->fwd_parts, s));
+      MBIR_TRY(mbir_cone_fwd_linesearch(j->delta, pix, P, j->view_table, j->cone_geom, nullptr,
+                                        j->weights_vcr, adelta, j->fwd_parts, s));
     n_fwd = mbir_cone_fwd_partials_count(j->cone_geom);
   }
   // The positivity clip changes delta after the line search, so A(delta)

@@ -759,7 +759,7 @@ class _VcdEngine:
         self.back_parts = torch.empty(back_cap, dtype=torch.float32, device=dev)
         self.delta = torch.empty(max_p * S, dtype=torch.float32, device=dev)
         n_dir = int(lib.mbir_vcd_direction_partials_count(max_p, S))
-        self.dir_parts = torch.empty(2 * n_dir, dtype=torch.float64, device=dev)
+        self.dir_parts = torch.empty(3 * n_dir, dtype=torch.float64, device=dev)
         n_fwd = max(ops.fwd_partials_count(ray=False), ops.fwd_partials_count(ray=True))
         self.fwd_parts = torch.empty(2 * n_fwd, dtype=torch.float64, device=dev)
         self.alpha = torch.zeros(1, dtype=torch.float32, device=dev)

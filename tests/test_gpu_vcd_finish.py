@@ -28,8 +28,8 @@ def _case(rng, P, S, n_sino, n_dir, n_fwd, npix=4096):
     d = rng.standard_normal((P, S)).astype(np.float32)
     dup = np.flatnonzero(pix[1:] == pix[:-1])
     d[dup + 1] = d[dup]  # a listed-twice pixel carries equal deltas
-    dir_parts = rng.standard_normal(2 * n_dir) * 0.1
-    dir_parts[1::2] = np.abs(dir_parts[1::2]) + 1.0
+    dir_parts = rng.standard_normal(3 * n_dir) * 0.1  # (prior lin, prior quad, back lin)
+    dir_parts[1::3] = np.abs(dir_parts[1::3]) + 1.0
     fwd_parts = rng.standard_normal(2 * n_fwd) * 0.1
     fwd_parts[1::2] = np.abs(fwd_parts[1::2]) + 1.0
     fwd_parts[0::2] += 0.5
@@ -147,3 +147,35 @@ def test_finish_bf16_and_read_alpha(n):
     ref = x0.clone()
     ref.index_add_(0, c['pix'], 0.25 * c['delta'])
     torch.testing.assert_close(c['x'], ref, rtol=0, atol=1e-6)
+
+
+def test_linear_term_from_back_projection():
+    """<W e, A delta> from the line search with the error sinogram equals
+    <A^T W e, delta> from the weighted back projection (the direction
+    partials' third value), and the line search without the error sinogram
+    forms the same quadratic term with a zero linear term."""
+    L = _lib()
+    lib, p, st = L.lib(), L.ptr, L.stream_ptr()
+    import mbirtorch_amd as mb
+    V, R, C = 60, 8, 48
+    model = mb.ParallelBeamModel((V, R, C), np.linspace(0, np.pi, V, endpoint=False))
+    ops = model.projector()
+    rs = tuple(model.get_params('recon_shape'))
+    rng = np.random.default_rng(3)
+    np.random.seed(3)
+    sub = torch.as_tensor(mb.gen_pixel_partition(rs, 8)[0].astype(np.int64), device='cuda')
+    P = int(sub.shape[0])
+    d = torch.as_tensor(rng.standard_normal((P, R)), dtype=torch.float32, device='cuda')
+    e = torch.as_tensor(rng.standard_normal((V, C, R)), dtype=torch.float32, device='cuda')
+    w = torch.as_tensor(rng.random((V, C, R)) + 0.5, dtype=torch.float32, device='cuda')
+    n = ops.fwd_partials_count()
+    pa = torch.empty(2 * n, dtype=torch.float64, device='cuda')
+    pb = torch.empty(2 * n, dtype=torch.float64, device='cuda')
+    ops.fwd_linesearch(d, sub, e, w, pa)
+    ops.fwd_linesearch(d, sub, None, w, pb)
+    bt = ops.back(e, sub, weights=w)  # A^T W e at the subset, (P, R)
+    torch.cuda.synchronize()
+    a, b = pa.view(n, 2).cpu().numpy(), pb.view(n, 2).cpu().numpy()
+    assert np.array_equal(a[:, 1], b[:, 1]) and not b[:, 0].any()
+    back_lin = float((bt.double() * d.double()).sum())
+    assert abs(a[:, 0].sum() - back_lin) <= 1e-5 * np.abs(a[:, 0]).sum()
