@@ -39,15 +39,15 @@ def test_conversion_rounds_to_nearest_even():
     assert np.array_equal(back, t.to(torch.bfloat16).float().cpu().numpy())
 
 
-@pytest.mark.parametrize('weighted', [False, True])
-def test_parallel_bf16_kernels_equal_f32_on_widened_values(weighted):
+@pytest.mark.parametrize('weighted,R', [(False, 8), (True, 8), (True, 32), (True, 64)])
+def test_parallel_bf16_kernels_equal_f32_on_widened_values(weighted, R):
     import mbirtorch_amd as mb
     from mbirtorch_amd import _lib
     from mbirtorch_amd.tomography_model import _to_bf16, _to_f32
-    print(f'[gpu] bf16 parallel kernels weighted={weighted}', flush=True)
-    m = _model()
+    print(f'[gpu] bf16 parallel kernels weighted={weighted} R={R}', flush=True)
+    V, C = 40, 48
+    m = _model(V=V, R=R, C=C)
     ops = m.projector()
-    V, R, C = 40, 8, 48
     rs = tuple(m.get_params('recon_shape'))
     g = torch.Generator(device='cuda').manual_seed(1)
     e16 = _to_bf16(torch.randn((V, C, R), device='cuda', generator=g))

@@ -90,7 +90,7 @@ def test_back_matches_oracle(variant, coeff_power):
     assert err < TOL_PROJ
 
 
-@pytest.mark.parametrize('R', [1, 3, 8, 16, 64, 80])
+@pytest.mark.parametrize('R', [1, 3, 8, 16, 20, 24, 32, 48, 64, 80])
 def test_row_counts(R):
     """Row widths below, at and above one wavefront (lane layouts / row tiles)."""
     from oracle import parallel as op
@@ -262,13 +262,21 @@ def test_autograd_finite_differences():
     assert worst < 5e-3
 
 
-@pytest.mark.parametrize('sino_shape', [(90, 16, 64), (37, 5, 50)])
-def test_forward_ray_and_binned_gathers_agree(sino_shape):
+@pytest.mark.parametrize('sino_shape,params', [
+    ((90, 16, 64), {}), ((37, 5, 50), {}), ((40, 24, 64), {}), ((36, 32, 96), {}),
+    ((30, 48, 64), {}), ((20, 64, 128), {}), ((24, 28, 64), {}),
+    ((30, 32, 64), dict(delta_voxel=2.0)),       # 2*psf+1 taps: n_p stored per entry
+    ((30, 32, 64), dict(det_channel_offset=2.37)),
+])
+def test_forward_ray_and_binned_gathers_agree(sino_shape, params):
     """The full-list (ray) and arbitrary-list (binned) forward gathers sum the
-    same weights; they differ only in summation order."""
+    same weights; they differ only in summation order.  Covers every lane
+    configuration of both forms (rows per chunk 8 .. 64, two channels or four
+    per unit) and the long-bin placement (a dense list at 90 degrees puts an
+    image row on one channel)."""
     import mbirtorch_amd as mb
-    print(f'[gpu] ray vs binned {sino_shape}', flush=True)
-    m, _ = _model(sino_shape)
+    print(f'[gpu] ray vs binned {sino_shape} {params}', flush=True)
+    m, _ = _model(sino_shape, **params)
     rs = m.get_params('recon_shape')
     idx = m.full_indices_device()
     vals = torch.rand((idx.shape[0], rs[2]), device='cuda')
