@@ -39,8 +39,9 @@ struct ConePixel {
 
 __device__ __forceinline__ void cone_pixel(int64_t pix, const ConeView& vw,
                                            const mbir_cone_geom& g, ConePixel& o) {
-  const int row = (int)(pix / g.num_cols);
-  const int col = (int)(pix - (int64_t)row * g.num_cols);
+  const int pj = (int)pix;  // pixel indices lie in [0, rows*cols) < 2^31
+  const int row = pj / g.num_cols;
+  const int col = pj - row * g.num_cols;
   const float y_t = g.delta_voxel_row * ((float)row - g.row_center);
   const float x_t = g.delta_voxel * ((float)col - g.col_center);
   const float x = vw.cosv * x_t - vw.sinv * y_t;

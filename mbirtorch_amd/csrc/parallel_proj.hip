@@ -69,8 +69,10 @@ template <int NT>
 __device__ __forceinline__ void par_taps(int64_t pix, const ParView& vw,
                                          const mbir_par_geom& g, int& base,
                                          float (&a)[NT]) {
-  const int row = (int)(pix / g.num_cols);
-  const int col = (int)(pix - (int64_t)row * g.num_cols);
+  // Pixel indices lie in [0, rows*cols) < 2^31: 32-bit division.
+  const int pj = (int)pix;
+  const int row = pj / g.num_cols;
+  const int col = pj - row * g.num_cols;
   const float n_p = par_np(row, col, vw, g);
   base = par_base<NT>(n_p, g);
 #pragma unroll
@@ -471,22 +473,6 @@ __device__ __forceinline__ void fwd_write_partials(double lin, double quad,
   if (threadIdx.x == 0) {
     partials[2 * b] = lin;
     partials[2 * b + 1] = quad;
-  }
-}
-
-// Geometry of one (view, pixel) for the forward forms: NT-wide tap window
-// (two-tap rule, or round(n_p) +- psf with `ntaps` = 2*psf+1 <= NT live taps).
-template <int NT, bool TWO>
-__device__ __forceinline__ void fwd_taps(int64_t pix, const ParView& vw, const mbir_par_geom& g,
-                                         int ntaps, int& base, float (&a)[NT]) {
-  const int row = (int)(pix / g.num_cols);
-  const int col = (int)(pix - (int64_t)row * g.num_cols);
-  const float n_p = par_np(row, col, vw, g);
-  base = par_base<TWO ? 2 : 0>(n_p, g);
-#pragma unroll
-  for (int k = 0; k < NT; ++k) {
-    const int n = base + k;
-    a[k] = (k < ntaps && n >= 0 && n < g.num_channels) ? par_weight(n_p, n, vw) : 0.0f;
   }
 }
 
