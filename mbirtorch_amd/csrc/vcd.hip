@@ -26,12 +26,41 @@ __device__ __forceinline__ float qg_b_tilde(float delta, const mbir_vcd_params& 
   return num / (den * den);
 }
 
+// Sum of the view-split back partials of element i, split by split.  With
+// G > 1 the G consecutive lanes of a group share element i: lane `sub` sums
+// splits sub, sub+G, ... in order and an xor butterfly combines the G sums
+// (commutative pairings, so every lane of the group holds the same bits) --
+// small subsets then wait for ns/G loads instead of ns.
+template <int G>
+__device__ __forceinline__ float split_sum(const float* __restrict__ back_parts, int nsplit,
+                                           int64_t total, int64_t i, int sub) {
+  float bsum = 0.0f;
+  int k = sub;
+  if (k < nsplit) {
+    bsum = back_parts[(size_t)k * total + i];
+    k += G;
+  }
+  // Loads go out in batches of eight so their latencies overlap.
+  for (; k + 7 * G < nsplit; k += 8 * G) {
+    float t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = back_parts[(size_t)(k + u * G) * total + i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) bsum += t[u];
+  }
+  for (; k < nsplit; k += G) bsum += back_parts[(size_t)k * total + i];
+#pragma unroll
+  for (int m = 1; m < G; m <<= 1) bsum += __shfl_xor(bsum, m, G);
+  return bsum;
+}
+
 // Prior gradient/Hessian (qGGMRF surrogate or proximal prior) and forward
 // gradient/Hessian at subset element i = (p, s).
+template <int G>
 __device__ __forceinline__ void vcd_terms(const float* __restrict__ x,
-                                          const int64_t* __restrict__ pix, int64_t i, int64_t total,
-                                          const float* __restrict__ back_parts, int nsplit,
-                                          const float* __restrict__ hess,
+                                          const int64_t* __restrict__ pix, int64_t i, int sub,
+                                          int64_t total, const float* __restrict__ back_parts,
+                                          int nsplit, const float* __restrict__ hess,
                                           const float* __restrict__ prox,
                                           const mbir_vcd_params& prm, float& fg, float& gp,
                                           float& fh, float& hp, float& bsum_out) {
@@ -63,23 +92,15 @@ __device__ __forceinline__ void vcd_terms(const float* __restrict__ x,
       hp = hp + c2;
     }
   }
-  // The split partials are added in split order; the loads go out in
-  // batches of eight so their latencies overlap.
-  float bsum = back_parts[i];
-  int k = 1;
-  for (; k + 8 <= nsplit; k += 8) {
-    float t[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) t[u] = back_parts[(size_t)(k + u) * total + i];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) bsum += t[u];
-  }
-  for (; k < nsplit; ++k) bsum += back_parts[(size_t)k * total + i];
+  const float bsum = split_sum<G>(back_parts, nsplit, total, i, sub);
   fg = prm.neg_fm_constant * bsum;
   fh = prm.fm_constant * hess[pi * S + s];
   bsum_out = bsum;
 }
 
+// G lanes per element (split_sum); lane 0 of each group writes delta and
+// accumulates the partials.
+template <int G>
 __global__ __launch_bounds__(kThreads) void vcd_direction_kernel(
     const float* __restrict__ x, const int64_t* __restrict__ pix, int P,
     const float* __restrict__ back_parts, int nsplit, const float* __restrict__ hess,
@@ -87,15 +108,19 @@ __global__ __launch_bounds__(kThreads) void vcd_direction_kernel(
     double* __restrict__ partials) {
   const int64_t total = (int64_t)P * prm.num_slices;
   double pl = 0.0, pq = 0.0, bl = 0.0, unused = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * kThreads) {
+  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total * G;
+       t += (int64_t)gridDim.x * kThreads) {
+    const int64_t i = t / G;
+    const int sub = (int)(t % G);
     float fg, gp, fh, hp, bs;
-    vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp, bs);
+    vcd_terms<G>(x, pix, i, sub, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp, bs);
     const float d = -((fg + gp) / (fh + hp));
-    delta[i] = d;
-    pl += (double)(gp * d);
-    pq += (double)((hp * d) * d);
-    bl += (double)(bs * d);
+    if (sub == 0) {
+      delta[i] = d;
+      pl += (double)(gp * d);
+      pq += (double)((hp * d) * d);
+      bl += (double)(bs * d);
+    }
   }
   block_sum2_f64(pl, pq);
   block_sum2_f64(bl, unused);
@@ -107,15 +132,19 @@ __global__ __launch_bounds__(kThreads) void vcd_direction_kernel(
 }
 
 // DC damping pass 1: g, h_inv and the prior terms per element.
+template <int G>
 __global__ __launch_bounds__(kThreads) void vcd_dc_terms_kernel(
     const float* __restrict__ x, const int64_t* __restrict__ pix, int P,
     const float* __restrict__ back_parts, int nsplit, const float* __restrict__ hess,
     const float* __restrict__ prox, mbir_vcd_params prm, float* __restrict__ scratch) {
   const int64_t total = (int64_t)P * prm.num_slices;
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * kThreads) {
+  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total * G;
+       t += (int64_t)gridDim.x * kThreads) {
+    const int64_t i = t / G;
+    const int sub = (int)(t % G);
     float fg, gp, fh, hp, bs;
-    vcd_terms(x, pix, i, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp, bs);
+    vcd_terms<G>(x, pix, i, sub, total, back_parts, nsplit, hess, prox, prm, fg, gp, fh, hp, bs);
+    if (sub != 0) continue;
     scratch[i] = fg + gp;                   // g
     scratch[total + i] = 1.0f / (fh + hp);  // h_inv
     scratch[2 * total + i] = gp;
@@ -480,9 +509,16 @@ using namespace mbir;
 
 extern "C" {
 
+// Lanes per element of the direction kernels: small subsets spread each
+// element's split sum over a group of lanes.
+static inline int dir_group(int32_t P, int32_t S) {
+  const int64_t n = (int64_t)P * S;
+  return n <= 16384 ? 8 : (n <= 65536 ? 4 : 1);
+}
+
 int32_t mbir_vcd_direction_partials_count(int32_t P, int32_t S) {
-  // One partial pair per workgroup; large subsets stride over at most 1024.
-  const int64_t nb = ((int64_t)P * S + kThreads - 1) / kThreads;
+  // One partial triple per workgroup; large subsets stride over at most 1024.
+  const int64_t nb = ((int64_t)P * S * dir_group(P, S) + kThreads - 1) / kThreads;
   return (int32_t)(nb > 1024 ? 1024 : nb);
 }
 
@@ -494,9 +530,20 @@ int mbir_vcd_direction(const float* x, const int64_t* pix, int32_t P, const floa
   if (prm->use_prox && !prox) return MBIR_ERR_ARG;
   if (P <= 0 || nsplit < 1 || prm->num_slices <= 0) return MBIR_ERR_SHAPE;
   const int nb = mbir_vcd_direction_partials_count(P, prm->num_slices);
-  vcd_direction_kernel<<<nb, kThreads, 0, (hipStream_t)stream>>>(x, pix, P, back_parts, nsplit,
-                                                                  hess, prox, *prm, delta,
-                                                                  partials);
+  hipStream_t s = (hipStream_t)stream;
+  switch (dir_group(P, prm->num_slices)) {
+    case 8:
+      vcd_direction_kernel<8><<<nb, kThreads, 0, s>>>(x, pix, P, back_parts, nsplit, hess, prox,
+                                                      *prm, delta, partials);
+      break;
+    case 4:
+      vcd_direction_kernel<4><<<nb, kThreads, 0, s>>>(x, pix, P, back_parts, nsplit, hess, prox,
+                                                      *prm, delta, partials);
+      break;
+    default:
+      vcd_direction_kernel<1><<<nb, kThreads, 0, s>>>(x, pix, P, back_parts, nsplit, hess, prox,
+                                                      *prm, delta, partials);
+  }
   MBIR_CHECK_LAUNCH();
   return MBIR_OK;
 }
@@ -522,8 +569,19 @@ int mbir_vcd_direction_dc(const float* x, const int64_t* pix, int32_t P, const f
   uintptr_t gaddr = reinterpret_cast<uintptr_t>(scratch + 5 * total);
   gaddr = (gaddr + 7) & ~(uintptr_t)7;
   double* gbar = reinterpret_cast<double*>(gaddr);
-  vcd_dc_terms_kernel<<<nb, kThreads, 0, s>>>(x, pix, P, back_parts, nsplit, hess, prox, *prm,
-                                               scratch);
+  switch (dir_group(P, S)) {
+    case 8:
+      vcd_dc_terms_kernel<8><<<nb, kThreads, 0, s>>>(x, pix, P, back_parts, nsplit, hess, prox,
+                                                     *prm, scratch);
+      break;
+    case 4:
+      vcd_dc_terms_kernel<4><<<nb, kThreads, 0, s>>>(x, pix, P, back_parts, nsplit, hess, prox,
+                                                     *prm, scratch);
+      break;
+    default:
+      vcd_dc_terms_kernel<1><<<nb, kThreads, 0, s>>>(x, pix, P, back_parts, nsplit, hess, prox,
+                                                     *prm, scratch);
+  }
   MBIR_CHECK_LAUNCH();
   vcd_dc_gbar_kernel<<<S, kThreads, 0, s>>>(scratch, P, S, gbar);
   MBIR_CHECK_LAUNCH();
