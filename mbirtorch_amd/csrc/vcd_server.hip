@@ -33,6 +33,7 @@ struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
+  bool waited = false;  // completion established by a stream wait, not the event
   Job job;
 };
 
@@ -52,14 +53,35 @@ struct mbir_server {
 
 namespace {
 
+// The job's last launch is queued: record its completion event.  If the
+// record fails the completion cannot be polled, so wait for the stream here.
+void mark_issued(Slot& sl) {
+  sl.job.issued = true;
+  sl.waited = false;
+  if (hipEventRecord(sl.done, sl.stream) != hipSuccess) {
+    if (hipStreamSynchronize(sl.stream) != hipSuccess && sl.job.status == MBIR_OK)
+      sl.job.status = MBIR_ERR_LAUNCH;
+    sl.waited = true;
+  }
+}
+
+// Whether an issued job has finished on the device (an event error ends the
+// job with MBIR_ERR_LAUNCH rather than leaving it pending).
+bool slot_done(Slot& sl) {
+  if (sl.waited) return true;
+  const hipError_t q = hipEventQuery(sl.done);
+  if (q == hipErrorNotReady) return false;
+  if (q != hipSuccess && sl.job.status == MBIR_OK) sl.job.status = MBIR_ERR_LAUNCH;
+  return true;
+}
+
 // Issue the next subset step of a job (or its end-of-iteration statistics
 // for an empty iteration); marks the job issued after its last step.
 void issue_step(Slot& sl) {
   Job& j = sl.job;
   const int S = j.job.prm.num_slices;
   if (j.iter >= j.num_iters || j.status != MBIR_OK) {
-    j.issued = true;
-    hipEventRecord(sl.done, sl.stream);
+    mark_issued(sl);
     return;
   }
   const int begin = j.iter_offsets[j.iter], end = j.iter_offsets[j.iter + 1];
@@ -74,14 +96,12 @@ void issue_step(Slot& sl) {
   } else {
     ++j.step;
   }
-  if (j.iter >= j.num_iters || j.status != MBIR_OK) {
-    j.issued = true;
-    hipEventRecord(sl.done, sl.stream);
-  }
+  if (j.iter >= j.num_iters || j.status != MBIR_OK) mark_issued(sl);
 }
 
 void worker_loop(mbir_server* sv) {
-  hipSetDevice(sv->device);
+  // On a device it cannot select, the worker fails every job it takes.
+  const bool device_ok = hipSetDevice(sv->device) == hipSuccess;
   for (;;) {
     {
       std::unique_lock<std::mutex> lk(sv->mu);
@@ -90,6 +110,7 @@ void worker_loop(mbir_server* sv) {
           sl.job = sv->pending.front();
           sv->pending.pop_front();
           sl.busy = true;
+          if (!device_ok) sl.job.status = MBIR_ERR_LAUNCH;
         }
       }
       bool any = false;
@@ -108,7 +129,7 @@ void worker_loop(mbir_server* sv) {
       }
     }
     for (Slot& sl : sv->slots) {
-      if (sl.busy && sl.job.issued && hipEventQuery(sl.done) == hipSuccess) {
+      if (sl.busy && sl.job.issued && slot_done(sl)) {
         std::lock_guard<std::mutex> lk(sv->mu);
         sv->finished.emplace_back(sl.job.tag, sl.job.status);
         sl.busy = false;
@@ -136,8 +157,8 @@ mbir_server* mbir_server_create(int32_t max_active) {
     if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
       for (Slot& s2 : sv->slots) {
-        if (s2.stream) hipStreamDestroy(s2.stream);
-        if (s2.done) hipEventDestroy(s2.done);
+        if (s2.stream) (void)hipStreamDestroy(s2.stream);
+        if (s2.done) (void)hipEventDestroy(s2.done);
       }
       delete sv;
       return nullptr;
@@ -204,8 +225,8 @@ void mbir_server_destroy(mbir_server* sv) {
   sv->cv.notify_all();
   sv->worker.join();
   for (Slot& sl : sv->slots) {
-    hipStreamDestroy(sl.stream);
-    hipEventDestroy(sl.done);
+    (void)hipStreamDestroy(sl.stream);
+    (void)hipEventDestroy(sl.done);
   }
   delete sv;
 }
