@@ -17,6 +17,14 @@ LIB_PATH = os.path.join(_HERE, 'libmbir_hip.so')
 MBIR_LAYOUT_VRC = 0
 MBIR_LAYOUT_VCR = 1
 
+# Status codes of include/mbir_hip.h.
+MBIR_OK = 0
+MBIR_ERR_SHAPE = -1
+MBIR_ERR_GEOMETRY = -2
+MBIR_ERR_LAUNCH = -3
+MBIR_ERR_WORKSPACE = -4
+MBIR_ERR_ARG = -5
+
 _ERRORS = {-1: (ValueError, 'shape mismatch'), -2: (ValueError, 'invalid geometry'),
            -3: (RuntimeError, 'kernel launch failed'),
            -4: (RuntimeError, 'workspace too small'), -5: (ValueError, 'invalid argument')}

@@ -212,13 +212,18 @@ class ReconServer:
         if n < 0:
             _lib.check(n, 'server poll')
         out = []
+        failed = []
         for i in range(n):
             tag, status = int(self._tags[i]), int(self._status[i])
             job = self.jobs.pop(tag)
-            _lib.check(status, f'request {tag}')
-            out.append(self._finish(job))
             self.ledger.release(job['need'])
+            if status != 0:
+                failed.append((tag, status))
+                continue
+            out.append(self._finish(job))
         self._admit()
+        for tag, status in failed:  # after the ledger and the queue are settled
+            _lib.check(status, f'request {tag}')
         return out
 
     def _finish(self, job):

@@ -207,3 +207,40 @@ def test_memory_ledger_closed_form_and_admission():
     assert led.try_reserve(500) and not led.try_reserve(400) and led.try_reserve(300)
     led.release(500)
     assert led.used == 300 and led.fits_alone(800) and not led.fits_alone(801)
+
+
+def test_runner_and_engine_reject_bad_arguments_before_any_launch():
+    """mbir_vcd_run validates the job, the step count and every subset id on
+    the host (it returns MBIR_ERR_ARG before a launch, so fake device pointers
+    are never touched); without a device the request engine is not created."""
+    import ctypes
+    lib = _lib.lib()
+    g = _lib.ParGeom()
+    job = _lib.VcdJob()
+    fake = ctypes.c_void_p(0x1000)
+    job.geometry = _lib.MBIR_GEOMETRY_PARALLEL
+    job.num_subsets = 2
+    job.sino_dtype = _lib.MBIR_DTYPE_F32
+    job.par_geom = ctypes.pointer(g)
+    job.prm.num_slices = 4
+    for name in ('view_table', 'flat_recon', 'error_vcr', 'hessian', 'back_parts', 'delta',
+                 'dir_parts', 'fwd_parts', 'alpha', 'apply_scratch', 'red_scratch'):
+        setattr(job, name, fake)
+    sizes = (ctypes.c_int32 * 2)(8, 8)
+    pix = (ctypes.c_void_p * 2)(0x2000, 0x3000)
+    job.subset_sizes = sizes
+    job.subset_pixels = ctypes.cast(pix, ctypes.POINTER(ctypes.c_void_p))
+    stats = ctypes.c_void_p(0x4000)
+    bad = (ctypes.c_int32 * 1)(2)  # subset id out of range
+    assert lib.mbir_vcd_run(ctypes.byref(job), ctypes.cast(bad, ctypes.c_void_p), 1, stats,
+                            0, None) == _lib.MBIR_ERR_ARG
+    neg = (ctypes.c_int32 * 1)(-1)
+    assert lib.mbir_vcd_run(ctypes.byref(job), ctypes.cast(neg, ctypes.c_void_p), 1, stats,
+                            0, None) == _lib.MBIR_ERR_ARG
+    assert lib.mbir_vcd_run(ctypes.byref(job), None, -1, stats, 0, None) == _lib.MBIR_ERR_ARG
+    assert lib.mbir_vcd_run(ctypes.byref(job), None, 0, None, 0, None) == _lib.MBIR_ERR_ARG
+    job.sino_dtype = 7
+    assert lib.mbir_vcd_run(ctypes.byref(job), None, 0, stats, 0, None) == _lib.MBIR_ERR_ARG
+    if not torch.cuda.is_available():
+        assert not lib.mbir_server_create(4)
+    assert not lib.mbir_server_create(0) and not lib.mbir_server_create(65)
