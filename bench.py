@@ -196,7 +196,7 @@ def cpu_baseline(angles, vol, sino, views):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1 '
+        print(json.dumps({'error': 'mbirtorch_amd runs on one GPU; --gpus must be 1 '
                                    '(multi-GPU sharding is designed, not built)'}))
         sys.exit(2)
     import torch
