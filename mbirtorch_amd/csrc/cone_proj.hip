@@ -230,8 +230,9 @@ constexpr int kConeShortBin = 24;
 
 template <int NH>
 struct ConeBinCfg {
-  static constexpr int CH = NH <= 3 ? 256 : 128;  // pixels per chunk
-  static constexpr int ENT = CH * NH;             // entry capacity
+  static constexpr int CH = NH <= 3 ? 1024 : 512;  // pixels per chunk
+  static constexpr int PPT = CH / kCBlock;         // pixels per thread
+  static constexpr int ENT = CH * NH;              // entry capacity
 };
 
 template <int LPC, int NH, int NK, int MODE, bool WEIGHTED, typename TS>
@@ -272,32 +273,32 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_bin_kernel(
   for (int k = 0; k < CPU; ++k) acc[k] = 0.0f;
 
   for (int pc = 0; pc < n_pix; pc += Cfg::CH) {
-    // Geometry of pixel pc + tid; its horizontal taps landing in the block count.
-    int bin[NH];
-    float a[NH];
-    {
+    // Geometry of pixels pc + tid + q*kCBlock; their horizontal taps landing
+    // in the block count.
+    int bin[Cfg::PPT][NH];
+    float a[Cfg::PPT][NH];
+    if (tid < kConeCB) s_cnt[tid] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Cfg::PPT; ++q) {
+      const int jl = tid + q * kCBlock;
       int base = 0;
-      ConePixel q = {0, 1, 0, 1, -1e9f, 1};
-      const bool valid = tid < Cfg::CH && pc + tid < n_pix;
-      if (valid) {
-        cone_pixel(pix[pc + tid], vw, g, q);
-        cone_htaps<NH>(q, g, base, a);
+      ConePixel cp = {0, 1, 0, 1, -1e9f, 1};
+      if (pc + jl < n_pix) {
+        cone_pixel(pix[pc + jl], vw, g, cp);
+        cone_htaps<NH>(cp, g, base, a[q]);
       } else {
 #pragma unroll
-        for (int t = 0; t < NH; ++t) a[t] = 0.0f;
+        for (int t = 0; t < NH; ++t) a[q][t] = 0.0f;
       }
-      if (tid < Cfg::CH) {
-        s_m0[tid] = q.m0;
-        s_wr[tid] = q.Wr;
-        s_mag[tid] = q.mag;
-      }
-      if (tid < kConeCB) s_cnt[tid] = 0;
-      __syncthreads();
+      s_m0[jl] = cp.m0;
+      s_wr[jl] = cp.Wr;
+      s_mag[jl] = cp.mag;
 #pragma unroll
       for (int t = 0; t < NH; ++t) {
         const int b = base + t - c0;
-        const bool in = a[t] != 0.0f && b >= 0 && b < kConeCB;
-        bin[t] = in ? b : -1;
+        const bool in = a[q][t] != 0.0f && b >= 0 && b < kConeCB;
+        bin[q][t] = in ? b : -1;
         if (in) atomicAdd(&s_cnt[b], 1);
       }
     }
@@ -318,12 +319,15 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_bin_kernel(
     if (tid < kConeCB) s_cnt[tid] = s_off[tid];  // placement cursor
     if (!__syncthreads_or(tid < kConeCB && mine > kConeShortBin)) {
 #pragma unroll
-      for (int t = 0; t < NH; ++t) {
-        const int b = bin[t];
-        if (b >= 0) {
-          const int slot = atomicAdd(&s_cnt[b], 1);
-          s_ej[slot] = tid;
-          s_ea[slot] = a[t];
+      for (int q = 0; q < Cfg::PPT; ++q) {
+#pragma unroll
+        for (int t = 0; t < NH; ++t) {
+          const int b = bin[q][t];
+          if (b >= 0) {
+            const int slot = atomicAdd(&s_cnt[b], 1);
+            s_ej[slot] = tid + q * kCBlock;
+            s_ea[slot] = a[q][t];
+          }
         }
       }
       __syncthreads();
@@ -344,30 +348,33 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_bin_kernel(
       }
     } else {
 #pragma unroll
-      for (int t = 0; t < NH; ++t) {
-        const int b = bin[t];
-        int rank, gsize;
-        bool leader;
-        wave_bin_rank(b, rank, gsize, leader);
-        if (leader) s_wcnt[wave][b] = gsize;
-        if (!__syncthreads_or(b >= 0)) continue;
-        if (b >= 0) {
-          int slot = s_cnt[b] + rank;
-          for (int w = 0; w < wave; ++w) slot += s_wcnt[w][b];
-          s_ej[slot] = tid;
-          s_ea[slot] = a[t];
-        }
-        __syncthreads();
-        if (tid < kConeCB) {
-          int add = 0;
+      for (int q = 0; q < Cfg::PPT; ++q) {
 #pragma unroll
-          for (int w = 0; w < NW; ++w) {
-            add += s_wcnt[w][tid];
-            s_wcnt[w][tid] = 0;
+        for (int t = 0; t < NH; ++t) {
+          const int b = bin[q][t];
+          int rank, gsize;
+          bool leader;
+          wave_bin_rank(b, rank, gsize, leader);
+          if (leader) s_wcnt[wave][b] = gsize;
+          if (!__syncthreads_or(b >= 0)) continue;
+          if (b >= 0) {
+            int slot = s_cnt[b] + rank;
+            for (int w = 0; w < wave; ++w) slot += s_wcnt[w][b];
+            s_ej[slot] = tid + q * kCBlock;
+            s_ea[slot] = a[q][t];
           }
-          s_cnt[tid] += add;
+          __syncthreads();
+          if (tid < kConeCB) {
+            int add = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+              add += s_wcnt[w][tid];
+              s_wcnt[w][tid] = 0;
+            }
+            s_cnt[tid] += add;
+          }
+          __syncthreads();
         }
-        __syncthreads();
       }
     }
     __syncthreads();
