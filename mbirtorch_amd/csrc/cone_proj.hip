@@ -76,9 +76,13 @@ __device__ __forceinline__ float trap(float d, float W) {
   return fminf(fmaxf((W + 1.0f) / 2.0f - fabsf(d), 0.0f), fminf(W, 1.0f));
 }
 
-__device__ __forceinline__ float inv_cos_phi(float mag, float z, float sdd) {
-  const float r = (mag * z) / sdd;
-  return sqrtf(1.0f + r * r);
+// 1/cos(phi) of the ray through a voxel at height z and magnification mag:
+// sqrt(1 + (mag*z/sdd)^2), with 1/sdd from the caller and the hardware
+// square root (both within about one ulp of the divided, correctly rounded
+// form; the forward and back projectors share this function).
+__device__ __forceinline__ float inv_cos_phi(float mag, float z, float inv_sdd) {
+  const float r = (mag * z) * inv_sdd;
+  return __builtin_amdgcn_sqrtf(1.0f + r * r);
 }
 
 // Horizontal taps: base = round(n_p) - psf, NH weights (zero off-detector).
@@ -127,6 +131,7 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
   const bool k_ok = k < S;
   const float kf = (float)k;
   const float zrel = g.delta_voxel_slice * (kf - g.slice_center);
+  const float inv_sdd = 1.0f / g.source_detector_dist;
   const int p_base = bx * PIX_WG;
   const int v_begin = by * views_per_split;
   const int v_end = min(v_begin + views_per_split, g.num_views);
@@ -172,7 +177,7 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
           const float m0 = s_rec[(1 + NH) * STRIDE + qd];
           const float Wr = s_rec[(2 + NH) * STRIDE + qd];
           const float mag = s_rec[(3 + NH) * STRIDE + qd];
-          const float ic = inv_cos_phi(mag, z, g.source_detector_dist);
+          const float ic = inv_cos_phi(mag, z, inv_sdd);
           const float m_p = m0 + Wr * kf;
           const int mc = (int)rintf(m_p) - g.psf_radius;
           float sum = 0.0f;
@@ -262,7 +267,7 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_bin_kernel(
   const bool row_ok = m < R;
   const float mf = (float)m;
   const ConeView vw = views[v];
-  const float sdd = g.source_detector_dist;
+  const float inv_sdd = 1.0f / g.source_detector_dist;
   if (tid < kConeCB) {
 #pragma unroll
     for (int w = 0; w < NW; ++w) s_wcnt[w][tid] = 0;
@@ -410,7 +415,7 @@ __global__ __launch_bounds__(kCBlock) void cone_fwd_bin_kernel(
               A = (ki >= 0 && ki < S) ? A : 0.0f;
               const int kcl = min(max(ki, 0), S - 1);
               const float z = g.delta_voxel_slice * ((float)kcl - g.slice_center) + vw.z_off;
-              const float gv = xp[kcl] * inv_cos_phi(mag, z, sdd);
+              const float gv = xp[kcl] * inv_cos_phi(mag, z, inv_sdd);
               det = det + A * gv;
             }
             acc[k] = acc[k] + s_ea[e] * det;
