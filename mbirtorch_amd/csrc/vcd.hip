@@ -13,13 +13,19 @@ namespace mbir {
 constexpr int kThreads = 256;
 constexpr float kEps32 = 1.1920928955078125e-07f;
 
+// a^b for a > 0 from the hardware base-2 logarithm and exponential (each
+// within about an ulp; the product's relative error is ~|b log2 a| * 2^-23).
+__device__ __forceinline__ float pow_pos(float a, float b) {
+  return __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(a));
+}
+
 // 2*b_tilde(delta) without the direction weight: rho'(delta)/delta of the
-// qGGMRF potential with |delta| floored at a_min (qggmrf.py:35-57).
+// qGGMRF potential with |delta| floored at a_min > 0 (qggmrf.py:35-57).
 __device__ __forceinline__ float qg_b_tilde(float delta, const mbir_vcd_params& p) {
   const float ad = fmaxf(fabsf(delta), p.a_min);
   const float ds = ad / p.t_sigma;
-  const float u = powf(ds, p.q_minus_p);
-  const float pm2 = powf(ad, p.p_minus_2);
+  const float u = pow_pos(ds, p.q_minus_p);
+  const float pm2 = pow_pos(ad, p.p_minus_2);
   float num = pm2 / p.two_sigma_p;
   num = (num * u) * (p.q_over_p + u);
   const float den = 1.0f + u;
