@@ -311,10 +311,6 @@ class TomographyModel(ParameterHandler):
     def auto_set_regularization_params(self, sinogram, weights=None):
         """sigma_y, sigma_x, sigma_prox from a <=20-view subsample (host numpy,
         tomography_model.py:2057-2227)."""
-        if torch.is_tensor(sinogram):
-            sinogram = sinogram.detach().cpu().numpy()
-        if torch.is_tensor(weights):
-            weights = weights.detach().cpu().numpy()
         if self.get_params('auto_regularize_flag'):
             small = self.subsample_views(sinogram)
             small_w = 1 if weights is None else self.subsample_views(weights)
@@ -357,9 +353,13 @@ class TomographyModel(ParameterHandler):
 
     @staticmethod
     def subsample_views(array, max_views_to_use=20):
+        """Every (n // m)-th view as a host array; a tensor is sliced where it
+        lives and only the sample is copied."""
         n = array.shape[0]
         m = min(max_views_to_use, n)
         step = max(n // m, 1)
+        if torch.is_tensor(array):
+            return array.detach()[::step].cpu().numpy()
         return np.array(array[::step])
 
     @staticmethod
@@ -587,22 +587,22 @@ class TomographyModel(ParameterHandler):
         seq = vcd_utils.gen_partition_sequence(self.get_params('partition_sequence'),
                                                max_iterations=max_iterations)
         seq = seq[first_iteration:]
-        sino_np = sinogram.detach().cpu().numpy() if torch.is_tensor(sinogram) \
-            else np.asarray(sinogram)
-        if np.iscomplexobj(sino_np):
+        # Input checks where the data live: a device tensor is checked on the
+        # device and only the auto-regularization's view sample is copied out.
+        sino_chk = sinogram.detach() if torch.is_tensor(sinogram) else np.asarray(sinogram)
+        if (sino_chk.is_complex() if torch.is_tensor(sino_chk) else np.iscomplexobj(sino_chk)):
             raise TypeError('sinogram must be real-valued; got complex dtype.')
-        if not np.isfinite(sino_np).all():
+        if not _all_finite(sino_chk):
             raise ValueError('sinogram contains NaN and/or Inf values.')
         if weights is not None:
-            w_np = weights.detach().cpu().numpy() if torch.is_tensor(weights) \
-                else np.asarray(weights)
-            if not np.isfinite(w_np).all():
+            w_chk = weights.detach() if torch.is_tensor(weights) else np.asarray(weights)
+            if not _all_finite(w_chk):
                 raise ValueError('weights contains NaN and/or Inf values.')
-            if (w_np < 0).any():
+            if bool((w_chk < 0).any()):
                 raise ValueError('weights contain negative values.')
-            if (w_np == 0).all():
+            if bool((w_chk == 0).all()):
                 raise ValueError('all weights are zero.')
-        reg = self.auto_set_regularization_params(sino_np, weights=weights)
+        reg = self.auto_set_regularization_params(sino_chk, weights=weights)
         return sinogram, weights, init_recon, partitions, seq, granularity, reg
 
     def recon(self, sinogram, weights=None, init_recon=None, max_iterations=15,
@@ -711,6 +711,13 @@ def _to_f32(t):
     _lib.check(_lib.lib().mbir_bf16_to_f32(_lib.ptr(t.contiguous()), _lib.ptr(out), t.numel(),
                                            _lib.stream_ptr()), 'to f32')
     return out
+
+
+def _all_finite(a):
+    """True when every entry is finite (a tensor is reduced on its device)."""
+    if torch.is_tensor(a):
+        return bool(torch.isfinite(a).all())
+    return bool(np.isfinite(a).all())
 
 
 class _VcdEngine:

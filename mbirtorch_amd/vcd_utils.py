@@ -41,11 +41,21 @@ def get_2d_ror_mask(recon_shape, *, use_ror_mask=True, crop_radius_pixels=0,
     return mask
 
 
-def gen_pixel_partition(recon_shape, num_subsets, use_ror_mask=True):
+def _ror_indices(recon_shape, use_ror_mask):
+    """Flat (row-major) indices of the region of reconstruction, int32."""
     nr, nc = recon_shape[:2]
     indices = np.arange(nr * nc, dtype=np.int32)
     if use_ror_mask is not False:
         indices = indices[get_2d_ror_mask(recon_shape, use_ror_mask=use_ror_mask).flatten() == 1]
+    return indices
+
+
+def gen_pixel_partition(recon_shape, num_subsets, use_ror_mask=True):
+    return _partition_of(_ror_indices(recon_shape, use_ror_mask), num_subsets)
+
+
+def _partition_of(indices, num_subsets):
+    """The subsets of one partition of `indices` (vcd_utils.py:97-157)."""
     if num_subsets > len(indices):
         num_subsets = len(indices)
         warnings.warn('\nThe number of partition subsets is greater than the number of '
@@ -63,9 +73,10 @@ def gen_pixel_partition(recon_shape, num_subsets, use_ror_mask=True):
 
 
 def gen_set_of_pixel_partitions(recon_shape, granularity, device=None, use_ror_mask=True):
-    return [torch.as_tensor(np.ascontiguousarray(
-        gen_pixel_partition(recon_shape, g, use_ror_mask=use_ror_mask)),
-        dtype=torch.int64, device=device) for g in granularity]
+    # The region's index list is the same for every granularity: formed once.
+    indices = _ror_indices(recon_shape, use_ror_mask)
+    return [torch.as_tensor(np.ascontiguousarray(_partition_of(indices, g)),
+                            dtype=torch.int64, device=device) for g in granularity]
 
 
 def gen_partition_sequence(partition_sequence, max_iterations):

@@ -128,6 +128,25 @@ def test_autoregularization_matches_reference_golden():
                     v = m.auto_set_regularization_params(sino, weights=weights)
             got = [v['sigma_y'], v['sigma_x'], v['sigma_prox']]
             np.testing.assert_allclose(got, g[f'{name}_{tag}'], rtol=1e-7)
+            # A tensor sinogram is view-sampled where it lives: same values.
+            m2 = mb.ParallelBeamModel(sino.shape, angles)
+            m2.set_params(sharpness=sharp)
+            with np.errstate(all='ignore'), warnings.catch_warnings():
+                warnings.simplefilter('ignore')
+                v2 = m2.auto_set_regularization_params(
+                    torch.as_tensor(sino), weights=None if weights is None
+                    else torch.as_tensor(weights))
+            assert [v2['sigma_y'], v2['sigma_x'], v2['sigma_prox']] == got
+
+
+def test_input_checks_on_tensors():
+    from mbirtorch_amd.tomography_model import _all_finite
+    a = torch.zeros(3, 4)
+    assert _all_finite(a) and _all_finite(a.numpy())
+    a[1, 2] = float('nan')
+    assert not _all_finite(a) and not _all_finite(a.numpy())
+    a[1, 2] = float('inf')
+    assert not _all_finite(a)
 
 
 @pytest.mark.skipif(torch.cuda.is_available(), reason='checks the no-GPU error path')
