@@ -252,7 +252,9 @@ int mbir_cone_fwd_update(const float* delta, const int64_t* pixel_indices,
 
 /* ---- VCD subset step --------------------------------------------------- */
 
-/* Number of FP64 partial pairs mbir_vcd_direction writes. */
+/* Number of FP64 partial triples mbir_vcd_direction (and _dc) writes.  Not
+ * monotonic in num_pixels: small subsets spread each element over more lanes
+ * and so over more workgroups. */
 int32_t mbir_vcd_direction_partials_count(int32_t num_pixels,
                                           int32_t num_slices);
 
@@ -386,7 +388,9 @@ int mbir_cone_fwd_update_bf16(const float* delta, const int64_t* pixel_indices,
  * its partition subsets (subset id -> device index array, size, optional
  * image-row run table).  Scratch sizes: back_parts >= back_parts_capacity
  * floats (max over subsets of splits * size * slices), delta >= max size *
- * slices floats, dir_parts >= 3 * mbir_vcd_direction_partials_count,
+ * slices floats, dir_parts >= 3 * the largest
+ * mbir_vcd_direction_partials_count(size, slices) over the subset sizes
+ * (small subsets use more workgroups: not always the largest subset's),
  * fwd_parts >= 2 * max forward partials count, apply_scratch >=
  * mbir_vcd_apply_scratch(max size, slices) doubles and zero before the first
  * run (it carries per-workgroup statistics to the end of each iteration,

@@ -713,6 +713,15 @@ def _to_f32(t):
     return out
 
 
+def _dir_partials_capacity(sizes, num_slices):
+    """Partial triples the direction step needs over subsets of these sizes:
+    the largest count, which need not be the largest subset's (smaller
+    subsets spread each element over more lanes and workgroups)."""
+    lib = _lib.lib()
+    return max(int(lib.mbir_vcd_direction_partials_count(int(P), int(num_slices)))
+               for P in set(sizes))
+
+
 def _all_finite(a):
     """True when every entry is finite (a tensor is reduced on its device)."""
     if torch.is_tensor(a):
@@ -765,7 +774,7 @@ class _VcdEngine:
         back_cap = max(ops.back_splits(P) * P * S for P in sizes)
         self.back_parts = torch.empty(back_cap, dtype=torch.float32, device=dev)
         self.delta = torch.empty(max_p * S, dtype=torch.float32, device=dev)
-        n_dir = int(lib.mbir_vcd_direction_partials_count(max_p, S))
+        n_dir = _dir_partials_capacity(sizes, S)
         self.dir_parts = torch.empty(3 * n_dir, dtype=torch.float64, device=dev)
         n_fwd = max(ops.fwd_partials_count(ray=False), ops.fwd_partials_count(ray=True))
         self.fwd_parts = torch.empty(2 * n_fwd, dtype=torch.float64, device=dev)

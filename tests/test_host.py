@@ -263,3 +263,17 @@ def test_runner_and_engine_reject_bad_arguments_before_any_launch():
     if not torch.cuda.is_available():
         assert not lib.mbir_server_create(4)
     assert not lib.mbir_server_create(0) and not lib.mbir_server_create(65)
+
+
+def test_direction_partials_capacity_covers_every_subset_size():
+    """The engine sizes the direction partials for the largest count over a
+    run's subset sizes, which at 128x128x8 is not the largest subset's."""
+    from mbirtorch_amd import _lib
+    from mbirtorch_amd.tomography_model import _dir_partials_capacity
+    lib = _lib.lib()
+    rs, S = (128, 128, 8), 8
+    sizes = [int(mb.gen_pixel_partition(rs, g).shape[1]) for g in (1, 2, 4, 8, 16, 32, 64, 128)]
+    counts = [int(lib.mbir_vcd_direction_partials_count(P, S)) for P in sizes]
+    cap = _dir_partials_capacity(sizes, S)
+    assert cap == max(counts) and all(c <= cap for c in counts)
+    assert max(counts) > counts[0]  # the largest subset alone would under-size it
