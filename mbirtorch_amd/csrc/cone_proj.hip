@@ -498,9 +498,9 @@ static inline int cone_fwd_lpc(int R) {
 }
 
 // Sinogram storage: float32, or bfloat16 bits (back with cp == 1, forward
-// line search / update).  Tap-window templates: 3 taps, or 9 (the widest
-// window; the trapezoid weights vanish outside the reference's narrower
-// windows, so a wider window gives the same sums).
+// line search / update).  Tap-window templates: 3, 5 or 9 taps, the
+// narrowest that holds the reference's 2*psf+1 (the trapezoid weights vanish
+// outside it, so the extra taps of a wider window add exact zeros).
 template <int RW, int NH, int NV>
 static int cone_back_launch3(const void* sino, const void* w, const int64_t* pix, int P,
                              const ConeView* views, const mbir_cone_geom& g, int cp, int ns,
@@ -603,8 +603,10 @@ static int cone_fwd_rc_sw(const ConeFwdArgs& a) {
 
 static int cone_fwd(const ConeFwdArgs& a) {
   if (a.bf16 && (a.mode == CF_WRITE_VCR || a.mode == CF_WRITE_VRC)) return MBIR_ERR_ARG;
-  const bool nh3 = 2 * a.g.psf_radius + 1 <= 3, nk3 = 2 * a.g.bp_psf_radius + 1 <= 3;
-  if (nh3) return nk3 ? cone_fwd_rc_sw<3, 3>(a) : cone_fwd_rc_sw<3, 9>(a);
+  const int nh = 2 * a.g.psf_radius + 1;
+  const bool nk3 = 2 * a.g.bp_psf_radius + 1 <= 3;
+  if (nh <= 3) return nk3 ? cone_fwd_rc_sw<3, 3>(a) : cone_fwd_rc_sw<3, 9>(a);
+  if (nh <= 5) return nk3 ? cone_fwd_rc_sw<5, 3>(a) : cone_fwd_rc_sw<5, 9>(a);
   return nk3 ? cone_fwd_rc_sw<9, 3>(a) : cone_fwd_rc_sw<9, 9>(a);
 }
 
@@ -626,8 +628,9 @@ int mbir::cone_back_any(const void* sino, const void* w, const int64_t* pix, int
   if (P == 0) return MBIR_OK;
   const ConeView* views = reinterpret_cast<const ConeView*>(view_table);
   hipStream_t s = (hipStream_t)stream;
-  if (2 * g->psf_radius + 1 <= 3)
-    return cone_back_taps<3>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
+  const int nt = 2 * g->psf_radius + 1;
+  if (nt <= 3) return cone_back_taps<3>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
+  if (nt <= 5) return cone_back_taps<5>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
   return cone_back_taps<9>(sino, w, pix, P, views, *g, cp, ns, out, s, bf16);
 }
 

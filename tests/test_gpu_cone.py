@@ -18,25 +18,32 @@ def _cone_case(kind, V=24, R=10, C=24):
     import mbirtorch_amd as mb
     from oracle import cone as oc
     angles = np.linspace(0, 2 * np.pi, V, endpoint=False) + 0.05
-    zs = np.linspace(-3, 3, V) if kind == 'helical' else None
+    zs = np.linspace(-3, 3, V) if kind in ('helical', 'thick') else None
     curved = kind == 'curved'
     sdd, sid = 80.0, 40.0
     m = mb.ConeBeamModel((V, R, C), angles, sdd, sid, helical_z_shifts=zs,
                          use_curved_detector=curved)
+    # 'thick': slices 2.5 voxels high, as in the helical mixed-precision
+    # geometry -- tap radius 2, the 5-tap windows.
+    vsa = 2.5 if kind == 'thick' else 1.0
+    if kind == 'thick':
+        m.set_params(voxel_slice_aspect=vsa)
     rs = tuple(m.get_params('recon_shape'))
     g = oc.geometry((V, R, C), angles, sdd, sid, recon_shape=rs,
                     delta_voxel=m.get_params('delta_voxel'), z_shifts=zs, curved=curved,
+                    voxel_slice_aspect=vsa,
                     recon_slice_offset=m.get_params('recon_slice_offset'))
     return m, g, rs
 
 
-@pytest.mark.parametrize('kind', ['flat', 'curved', 'helical'])
+@pytest.mark.parametrize('kind', ['flat', 'curved', 'helical', 'thick'])
 def test_cone_forward_back_match_oracle(kind):
     from oracle import cone as oc
     from oracle import parallel as op
     print(f'[gpu] cone parity {kind}', flush=True)
     m, g, rs = _cone_case(kind)
     assert m.get_psf_radii() == oc.psf_radii(g)
+    assert m.get_psf_radii()[0] == (2 if kind == 'thick' else 1)
     idx = op.full_indices(rs)
     rng = np.random.default_rng(0)
     vol = rng.random(rs).astype(np.float32)
