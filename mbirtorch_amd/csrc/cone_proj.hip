@@ -98,8 +98,25 @@ __device__ __forceinline__ void cone_htaps(const ConePixel& q, const mbir_cone_g
   }
 }
 
+// Orders a wavefront's LDS writes before its later LDS reads (lanes of one
+// wavefront exchanging values; no other wavefront takes part).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+constexpr int kConeRowCap = 256;  // detector rows of one wavefront's shared pass
+
 // ---------------------------------------------------------------------------
 // Back projection.  LDS record per (view, pixel): base, NH weights, m0, Wr, mag.
+// With one pixel per wavefront (64 slices on the lanes), when weighted or
+// with windows wider than 3 x 3, the slices of a (pixel, view) share detector
+// rows: the wavefront first forms each row's
+// horizontal sum det(m) = sum_t a_t * e(base + t, m) once (lanes = rows,
+// coalesced) into LDS, then every slice lane adds its vertical taps
+// A_v(m) * det(m) from there -- the same float operations in the same order
+// as forming det(m) per slice, so the sums are bit-identical.
 template <int RW, int NH, int NV, int CP, bool WEIGHTED, typename TS>
 __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
     const TS* __restrict__ sino, const TS* __restrict__ wts,
@@ -111,8 +128,13 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
   constexpr int PIX_WG = (kCBlock / MBIR_WAVE) * PIX_WAVE;
   constexpr int VB = PIX_WG >= 512 ? 1 : 512 / PIX_WG;
   constexpr int REC = NH + 4;
+  // The shared-row pass pays where a slice's own loop would load more than
+  // nine values per (pixel, view): weighted, or windows wider than 3 x 3
+  // (measured: 3 x 3 unweighted is faster per slice).
+  constexpr bool ROWS = SLOTS == 1 && (WEIGHTED || NH * NV > 9);
   __shared__ float s_rec[REC * VB * PIX_WG];
   __shared__ float s_zoff[VB];
+  __shared__ float s_det[ROWS ? kCBlock / MBIR_WAVE : 1][ROWS ? kConeRowCap : 1];
 
   const int lane = threadIdx.x & (MBIR_WAVE - 1), wave = threadIdx.x / MBIR_WAVE;
   const int slot = lane / RW, kl = lane % RW;
@@ -166,7 +188,9 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
       s_rec[(3 + NH) * STRIDE + qd] = q.mag;
     }
     __syncthreads();
-    if (k_ok) {
+    // Lane of the last slice below S (wave-uniform; the shared-row pass).
+    const int last = min(RW, S - bz * RW) - 1;
+    if (ROWS || k_ok) {
       for (int b = 0; b < nvb; ++b) {
         const float z = zrel + s_zoff[b];
         const size_t vofs = (size_t)(vb0 + b) * C * R;
@@ -181,6 +205,44 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
           const float m_p = m0 + Wr * kf;
           const int mc = (int)rintf(m_p) - g.psf_radius;
           float sum = 0.0f;
+          bool shared_rows = false;
+          if constexpr (ROWS) {
+            // mc grows with the slice (Wr > 0): rows lo .. hi cover every
+            // lane's taps.  Other spans take the per-slice loop below.
+            const int lo = __shfl(mc, 0, MBIR_WAVE);
+            const int cnt = __shfl(mc, last, MBIR_WAVE) + NV - lo;
+            if (cnt >= NV && cnt <= kConeRowCap) {
+              shared_rows = true;
+              wave_lds_sync();  // the previous pixel's reads are done
+              for (int j = lane; j < cnt; j += MBIR_WAVE) {
+                const int mcl = min(max(lo + j, 0), R - 1);
+                float det = 0.0f;
+#pragma unroll
+                for (int th = 0; th < NH; ++th) {
+                  const float ah = s_rec[(1 + th) * STRIDE + qd];
+                  const int n = min(max(base + th, 0), C - 1);
+                  float e = ld_s(sino + vofs + (size_t)n * R + mcl);
+                  if (WEIGHTED) e = e * ld_s(wts + vofs + (size_t)n * R + mcl);
+                  det = fmaf(ah, e, det);
+                }
+                s_det[wave][j] = det;
+              }
+              wave_lds_sync();
+              if (k_ok) {
+#pragma unroll
+                for (int tv = 0; tv < NV; ++tv) {
+                  const int mm = mc + tv;
+                  float Av = fminf(fmaxf((Wr + 1.0f) / 2.0f - fabsf(m_p - (float)mm), 0.0f),
+                                   fminf(Wr, 1.0f)) * ic;
+                  Av = (mm >= 0 && mm < R) ? Av : 0.0f;
+                  if (CP == 2) Av = Av * Av;
+                  const int j = min(max(mm - lo, 0), cnt - 1);
+                  sum = fmaf(Av, s_det[wave][j], sum);
+                }
+              }
+            }
+          }
+          if (!shared_rows && k_ok) {
 #pragma unroll
           for (int tv = 0; tv < NV; ++tv) {
             const int mm = mc + tv;
@@ -199,6 +261,7 @@ __global__ __launch_bounds__(kCBlock) void cone_back_kernel(
               det = fmaf(ah, e, det);
             }
             sum = fmaf(Av, det, sum);
+          }
           }
           acc[i] += sum;
         }
